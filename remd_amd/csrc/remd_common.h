@@ -1,0 +1,95 @@
+// remd_common.h -- definitions shared by the HIP kernels and the host engine.
+//
+// Device memory layout (one REMD job = one set of replicas of one system):
+//   molecule block, per replica: NFIELD arrays of nmol doubles (SoA, sorted
+//     molecule order): r[3] (fractional), v[3] (velocity*L*s), f[3],
+//     q[4] (quaternion), p[4] (quaternion momentum * s), n[4] (torque in the
+//     quaternion-4 form of src/molecules.cxx:409-418);
+//   mid: nmol ints, molecule identity permuted with the sort;
+//   scalar block, per replica: NSC doubles (thermostat, barostat, box,
+//     conditions and the properties of src/variables.h:6-49).
+#pragma once
+
+#include <stdint.h>
+
+namespace remd {
+
+constexpr int MAXSITE = 5;
+
+// molecule block field offsets (in units of nmol doubles)
+enum Field {
+  F_RX = 0, F_RY, F_RZ,
+  F_VX, F_VY, F_VZ,
+  F_FX, F_FY, F_FZ,
+  F_Q0, F_Q1, F_Q2, F_Q3,
+  F_P0, F_P1, F_P2, F_P3,
+  F_N0, F_N1, F_N2, F_N3,
+  NFIELD
+};
+
+// replica scalar block (doubles)
+enum Scalar {
+  S_S = 0, S_PS, S_Q,
+  S_PV0, S_PV1, S_PV2, S_W,
+  S_L0, S_L1, S_L2,
+  S_T, S_P,            // current conditions (reduced)
+  S_TNEW, S_PNEW,      // conditions requested for the next interval
+  S_XFLAG,             // 1.0: the replica was exchanged; rescale first
+  S_GKT, S_H0,
+  S_LJ, S_CLMB, S_WALL, S_POT,
+  S_VIR0, S_VIR1, S_VIR2,
+  S_TRA0, S_TRA1, S_TRA2, S_TRA, S_ROT, S_KIN,
+  S_TSM, S_TSP, S_TST, S_BST, S_TOT, S_HAM, S_TK,
+  S_PRS0, S_PRS1, S_PRS2, S_PRS3,
+  S_VOL,
+  S_STEPS,             // steps integrated so far (diagnostic)
+  NSC
+};
+
+// kernel modes
+enum Mode {
+  MODE_INIT = 0,       // sort (margin 0) + force + properties, H0 = tot
+  MODE_INTERVAL = 1,   // [exchange rescale] + interval steps + properties
+  MODE_FORCE = 2,      // force pass only (test hook), optional sort
+};
+
+struct JobParams {
+  int nmol, nsite, qbeg;        // qbeg: first charged site (0: 3-site, 1: 4/5-site)
+  int tmode, pmode, cmode;      // tconstant, pconstant, confined
+  int sort_axis;                // 2 (z) bulk/tube, 0 (x) slit
+  int per[3];                   // periodic axes
+  int interval, sort_interval;
+  int nfwall;
+  int chunks;                   // j-chunks per i-molecule in the pair pass (power of two)
+  int precision;                // 0 mixed (FP32 pair, FP64 acc), 1 FP64
+  int pad0;
+  double dt, dthalf;
+  double rcut, rswitch, margin;
+  double wall_length;           // A
+  double bottom_area;           // pi (R - 1.7)^2 for the tube
+  double mass, inertia[3];
+  double site_r[MAXSITE][3];    // body-frame site coordinates
+  double q[MAXSITE];            // reduced charges
+  double lj_sig2, lj_eps4;      // site 0 LJ: sigma^2, 4 epsilon
+  double unit_temp, unit_press;
+  const double *fwall;          // nfwall x {f, e}
+};
+
+struct ReplicaDesc {
+  const JobParams *job;         // device pointer
+  double *mol;                  // NFIELD * nmol doubles
+  int *mid;                     // nmol
+  double *sc;                   // NSC doubles
+  double *site_f;               // optional: nsite*nmol*3 (last force pass)
+  int *win;                     // optional: 2*nmol window (js, je)
+};
+
+struct LaunchArgs {
+  const ReplicaDesc *reps;
+  int nrep;
+  int mode;
+  int do_sort;                  // MODE_FORCE only
+  double margin;                // MODE_FORCE only
+};
+
+}  // namespace remd

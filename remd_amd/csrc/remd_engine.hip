@@ -1,0 +1,987 @@
+// remd_engine.hip -- host side of the C ABI in include/remd_gpu.h.
+//
+// A remd_gpu handle owns one REMD job's device-resident state (the
+// reference's REMDGPU, src/remdgpu.h:92-238): molecule blocks, scalar blocks,
+// the wall table and the per-replica descriptors the persistent kernel walks.
+// A remd_batch packs the replicas of several handles into one launch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/remd_gpu.h"
+#include "remd_common.h"
+
+namespace remd {
+hipError_t launch_interval(const LaunchArgs &a, int nt, int nsite, int prec, int lds, hipStream_t st);
+hipError_t set_kernel_lds_limit(int bytes);
+hipError_t launch_wall_tube(int nfwall, double R, double sgm06, double coefE, double *fw, hipStream_t st);
+hipError_t launch_wall_slit(int nfwall, double H, double sgm, double eps, double rho, double *fw,
+                            hipStream_t st);
+hipError_t launch_sort_window(const double *keys, int nrow, int n, double w, int *order, int *js, int *je,
+                              hipStream_t st);
+struct ExchangeJob {
+  int ntemp, npress, pmode, ec_type;
+  int *index;
+  double *T, *P;
+  const double *u;
+  double *const *sc;
+  int *accepted;
+};
+hipError_t launch_exchange(const ExchangeJob *jobs, int njobs, hipStream_t st);
+int lds_bytes_host(int nmol, int nsite, int prec, int nt);
+}  // namespace remd
+
+using namespace remd;
+
+// ------------------------------------------------------------ errors
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                       \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return fail(REMD_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));           \
+  } while (0)
+
+extern "C" const char *remd_gpu_last_error(void) { return g_err.c_str(); }
+extern "C" int remd_gpu_abi_version(void) { return REMD_GPU_ABI_VERSION; }
+
+// ------------------------------------------------------------- units
+// src/unit.h:3-18
+static const double kNa = 6.0221367e+23, kKB = 1.3806488e-23;
+static double unit_energy() { return 1e3 / kNa; }
+static double unit_temp() { return unit_energy() / kKB; }
+static double unit_press() { return unit_energy() / (1e-10 * 1e-10 * 1e-10); }
+
+namespace remd {
+static inline int align16h(int x) { return (x + 15) & ~15; }
+int lds_bytes_host(int nmol, int nsite, int prec, int nt) {
+  const int stride = (nsite == 5) ? 18 : (nsite == 4 ? 16 : 12);
+  const int rec = nmol * stride * (prec ? 8 : 4);
+  const int sortb = 20 * nmol;
+  const int region0 = align16h(rec > sortb ? rec : sortb);
+  const int win = align16h(4 * nmol);
+  const int misc = (nt / 64) * 32 * 8 + (16 * 64 + 16) * 4;
+  return region0 + win + align16h(misc);
+}
+}  // namespace remd
+
+static int device_lds_limit() {
+  static int lim = -1;
+  if (lim < 0) {
+    int dev = 0;
+    hipGetDevice(&dev);
+    int v = 0, o = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+      v = 65536;
+    if (hipDeviceGetAttribute(&o, hipDeviceAttributeSharedMemPerBlockOptin, dev) == hipSuccess && o > v) v = o;
+    lim = v;
+    set_kernel_lds_limit(lim);
+  }
+  return lim;
+}
+
+// -------------------------------------------------------------- job
+struct remd_gpu {
+  remd_gpu_desc desc;
+  JobParams prm;
+  int nrep = 0, N = 0, NS = 0;
+  int nt = 256, lds = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+  // device
+  JobParams *d_job = nullptr;
+  double *d_mol = nullptr;
+  int *d_mid = nullptr;
+  double *d_sc = nullptr;
+  double *d_fwall = nullptr;
+  double *d_site = nullptr;
+  int *d_win = nullptr;
+  ReplicaDesc *d_reps = nullptr;
+  // host
+  std::vector<ReplicaDesc> h_reps;
+  double *h_sc = nullptr;  // pinned mirror of the scalar blocks
+  bool debug = false;
+  // device exchange buffers
+  int *d_index = nullptr;
+  double *d_T = nullptr, *d_P = nullptr, *d_u = nullptr;
+  double **d_scptr = nullptr;
+  int *d_acc = nullptr;
+  ExchangeJob *d_xjob = nullptr;
+};
+
+// Kernel class = (sites, pair precision); one launch per class, each on its
+// own stream so classes run concurrently.
+constexpr int NCLASS = 6;
+static int kclass(int nsite, int prec) { return (nsite - 3) * 2 + prec; }
+
+struct remd_batch {
+  std::vector<remd_gpu *> jobs;
+  ReplicaDesc *d_table = nullptr;
+  int table_cap = 0;
+  int ntab = 0;
+  bool dirty = true;
+  int nt = 256;
+  int cls_off[NCLASS] = {0}, cls_n[NCLASS] = {0}, cls_lds[NCLASS] = {0};
+  hipStream_t stream = nullptr;
+  hipStream_t cstream[NCLASS] = {nullptr};
+  hipEvent_t cdone[NCLASS] = {nullptr};
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+};
+
+// Pair-pass chunk count: balance passes over the workgroup against the
+// per-lane window length (estimated from density along the sort axis).
+static int pick_chunks(int N, int nt, double window_frac) {
+  const double wcount = std::max(1.0, window_frac * N);
+  int best = 1;
+  double best_cost = 1e300;
+  for (int C = 1; C <= 16; C *= 2) {
+    const int passes = (N * C + nt - 1) / nt;
+    const double cost = passes * std::ceil(wcount / C) + 6.0 * std::log2((double)C) * passes;
+    if (cost < best_cost - 1e-9) {
+      best_cost = cost;
+      best = C;
+    }
+  }
+  return best;
+}
+
+static int validate(const remd_gpu_desc *d, const remd_gpu_scalars *sc) {
+  if (!d || !sc) return fail(REMD_EINVAL, "null descriptor");
+  if (d->nsite < 3 || d->nsite > 5) return fail(REMD_EINVAL, "nsite must be 3, 4 or 5");
+  if (d->nmol < 2 || d->nmol > 4096) return fail(REMD_EINVAL, "nmol must be in [2, 4096]");
+  if (d->nreplica < 1) return fail(REMD_EINVAL, "nreplica must be >= 1");
+  if (d->tconstant < 0 || d->tconstant > 2) return fail(REMD_EINVAL, "tconstant must be 0, 1 or 2");
+  if (d->confined < 0 || d->confined > 2) return fail(REMD_EINVAL, "confined must be 0, 1 or 2");
+  // valid ensemble/confinement pairs (src/remd.cxx:35-36 and GetBottomArea)
+  const int P = d->pconstant, Cm = d->confined;
+  const bool okp = (P == 0) || (Cm == 0 && P == 1) || (Cm == 1 && P == 2) || (Cm == 2 && P == 3);
+  if (!okp) return fail(REMD_EINVAL, "pconstant not valid for this confinement (bulk:1, tube:2, slit:3)");
+  if (d->interval < 0) return fail(REMD_EINVAL, "interval must be >= 0");
+  if (d->sort_interval < 1) return fail(REMD_EINVAL, "sort_interval must be >= 1");
+  if (!(d->rcut > d->rswitch) || d->rswitch <= 0) return fail(REMD_EINVAL, "need 0 < rswitch < rcut");
+  if (Cm > 0 && d->nfwall < 3) return fail(REMD_EINVAL, "confined systems need nfwall >= 3");
+  if (Cm > 0 && !(d->wall_length > 0)) return fail(REMD_EINVAL, "wall_length must be > 0");
+  if (!(d->site_eps[0] > 0)) return fail(REMD_EINVAL, "site 0 must carry the LJ interaction");
+  for (int a = 1; a < d->nsite; a++)
+    if (d->site_eps[a] != 0.0) return fail(REMD_EINVAL, "only site 0 may carry LJ parameters");
+  if (d->nsite >= 4 && d->site_q[0] != 0.0)
+    return fail(REMD_EINVAL, "4/5-site models must have an uncharged site 0");
+  if (d->block_threads != 0 && d->block_threads != 128 && d->block_threads != 256 && d->block_threads != 512)
+    return fail(REMD_EINVAL, "block_threads must be 0, 128, 256 or 512");
+  if (d->chunks != 0 && (d->chunks < 1 || d->chunks > 64 || (d->chunks & (d->chunks - 1))))
+    return fail(REMD_EINVAL, "chunks must be 0 or a power of two <= 64");
+  if (d->precision != 0 && d->precision != 1) return fail(REMD_EINVAL, "precision must be 0 or 1");
+  const int axis = (Cm == 2) ? 0 : 2;
+  for (int r = 0; r < d->nreplica; r++) {
+    const double *L = sc[r].L;
+    if (!(sc[r].s > 0) || !(sc[r].Q > 0) || !(sc[r].W > 0)) return fail(REMD_EINVAL, "s, Q and W must be > 0");
+    if (!(sc[r].T > 0)) return fail(REMD_EINVAL, "T must be > 0");
+    if (!(L[axis] > 2.0 * (d->rcut + d->margin)))
+      return fail(REMD_EINVAL, "box along the sort axis must exceed 2*(rcut+margin)");
+    const int per[3] = {Cm == 0 || Cm == 2, Cm == 0 || Cm == 2, Cm == 0 || Cm == 1};
+    for (int k = 0; k < 3; k++)
+      if (per[k] && !(L[k] > 2.0 * d->rcut)) return fail(REMD_EINVAL, "periodic box length must exceed 2*rcut");
+  }
+  return 0;
+}
+
+static void fill_params(const remd_gpu_desc *d, JobParams &J) {
+  memset(&J, 0, sizeof(J));
+  J.nmol = d->nmol;
+  J.nsite = d->nsite;
+  J.qbeg = (d->nsite == 3) ? 0 : 1;
+  J.tmode = d->tconstant;
+  J.pmode = d->pconstant;
+  J.cmode = d->confined;
+  J.sort_axis = (d->confined == 2) ? 0 : 2;
+  J.per[0] = J.per[1] = (d->confined == 0 || d->confined == 2);
+  J.per[2] = (d->confined == 0 || d->confined == 1);
+  J.interval = d->interval;
+  J.sort_interval = d->sort_interval;
+  J.nfwall = d->nfwall;
+  J.precision = d->precision;
+  J.dt = d->dt;
+  J.dthalf = d->dt * 0.5;
+  J.rcut = d->rcut;
+  J.rswitch = d->rswitch;
+  J.margin = d->margin;
+  J.wall_length = d->wall_length;
+  {
+    const double sigma_c = 3.4;  // src/molecules.h:369-371
+    const double wl = (d->wall_length - 0.5 * sigma_c);
+    J.bottom_area = M_PI * wl * wl;
+  }
+  double m = 0;
+  for (int a = 0; a < d->nsite; a++) m += d->site_m[a];
+  J.mass = d->mass > 0 ? d->mass : m;
+  for (int k = 0; k < 3; k++) J.inertia[k] = d->inertia[k];
+  for (int a = 0; a < d->nsite; a++) {
+    for (int k = 0; k < 3; k++) J.site_r[a][k] = d->site_r[a][k];
+    J.q[a] = d->site_q[a];
+  }
+  // MakeParamList diagonal (src/force_calculator.cxx:226-228): sigma of the
+  // pair = (s_i + s_j)/2, epsilon = sqrt(e_i e_j); 4 eps as the tuned kernel
+  const double sg = 0.5 * (d->site_sigma[0] + d->site_sigma[0]);
+  J.lj_sig2 = sg * sg;
+  J.lj_eps4 = 4.0 * std::sqrt(d->site_eps[0] * d->site_eps[0]);
+  J.unit_temp = unit_temp();
+  J.unit_press = unit_press();
+}
+
+extern "C" size_t remd_gpu_footprint(const remd_gpu_desc *d) {
+  if (!d) return 0;
+  const size_t N = (size_t)d->nmol, R = (size_t)d->nreplica;
+  size_t b = R * NFIELD * N * sizeof(double) + R * N * sizeof(int) + R * NSC * sizeof(double);
+  b += (size_t)std::max(d->nfwall, 0) * 2 * sizeof(double) + R * sizeof(ReplicaDesc) + sizeof(JobParams);
+  b += R * (sizeof(double) * 3 + sizeof(int) * 2 + sizeof(double *)) + 4096;
+  return b;
+}
+
+static void free_job(remd_gpu *g) {
+  if (!g) return;
+  hipFree(g->d_job);
+  hipFree(g->d_mol);
+  hipFree(g->d_mid);
+  hipFree(g->d_sc);
+  hipFree(g->d_fwall);
+  hipFree(g->d_site);
+  hipFree(g->d_win);
+  hipFree(g->d_reps);
+  hipFree(g->d_index);
+  hipFree(g->d_T);
+  hipFree(g->d_P);
+  hipFree(g->d_u);
+  hipFree(g->d_scptr);
+  hipFree(g->d_acc);
+  hipFree(g->d_xjob);
+  if (g->h_sc) hipHostFree(g->h_sc);
+  if (g->ev0) hipEventDestroy(g->ev0);
+  if (g->ev1) hipEventDestroy(g->ev1);
+  if (g->stream) hipStreamDestroy(g->stream);
+  delete g;
+}
+
+static int build_wall(remd_gpu *g) {
+  const remd_gpu_desc &d = g->desc;
+  if (d.confined == 0) return 0;
+  HIPCHK(hipMalloc(&g->d_fwall, sizeof(double) * 2 * d.nfwall));
+  if (d.confined == 1) {
+    // float powf of sigma as the reference (force_calculator.cxx:123)
+    const double sgm06 = (double)powf((float)d.sgm_wall, 6.0f);
+    const double coefE = M_PI * d.rho_wall * d.eps_wall * sgm06;
+    HIPCHK(launch_wall_tube(d.nfwall, d.wall_length, sgm06, coefE, g->d_fwall, g->stream));
+  } else {
+    HIPCHK(launch_wall_slit(d.nfwall, d.wall_length, d.sgm_wall, d.eps_wall, d.rho_wall, g->d_fwall, g->stream));
+  }
+  return 0;
+}
+
+static void refresh_reps(remd_gpu *g) {
+  g->h_reps.resize(g->nrep);
+  for (int r = 0; r < g->nrep; r++) {
+    ReplicaDesc &R = g->h_reps[r];
+    R.job = g->d_job;
+    R.mol = g->d_mol + (size_t)r * NFIELD * g->N;
+    R.mid = g->d_mid + (size_t)r * g->N;
+    R.sc = g->d_sc + (size_t)r * NSC;
+    R.site_f = g->debug ? g->d_site + (size_t)r * g->N * g->NS * 3 : nullptr;
+    R.win = g->debug ? g->d_win + (size_t)r * g->N * 2 : nullptr;
+  }
+}
+
+static int upload_reps(remd_gpu *g) {
+  refresh_reps(g);
+  HIPCHK(hipMemcpyAsync(g->d_reps, g->h_reps.data(), sizeof(ReplicaDesc) * g->nrep, hipMemcpyHostToDevice,
+                        g->stream));
+  return 0;
+}
+
+static int launch(remd_gpu *g, int mode, int do_sort, double margin, const ReplicaDesc *reps, int nrep,
+                  bool timed) {
+  LaunchArgs a;
+  a.reps = reps;
+  a.nrep = nrep;
+  a.mode = mode;
+  a.do_sort = do_sort;
+  a.margin = margin;
+  if (timed) HIPCHK(hipEventRecord(g->ev0, g->stream));
+  HIPCHK(launch_interval(a, g->nt, g->NS, g->prm.precision, g->lds, g->stream));
+  if (timed) HIPCHK(hipEventRecord(g->ev1, g->stream));
+  return 0;
+}
+
+static int sync_scalars(remd_gpu *g) {
+  HIPCHK(hipMemcpyAsync(g->h_sc, g->d_sc, sizeof(double) * NSC * g->nrep, hipMemcpyDeviceToHost, g->stream));
+  HIPCHK(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+extern "C" int remd_gpu_create(remd_gpu **out, const remd_gpu_desc *d, const remd_gpu_scalars *sc,
+                               const double *r, const double *v, const double *q, const double *p,
+                               const int *ids) {
+  if (!out) return fail(REMD_EINVAL, "null output handle");
+  *out = nullptr;
+  int rc = validate(d, sc);
+  if (rc) return rc;
+  if (!r || !v || !q || !p) return fail(REMD_EINVAL, "null state array");
+  remd_gpu *g = new remd_gpu;
+  g->desc = *d;
+  g->nrep = d->nreplica;
+  g->N = d->nmol;
+  g->NS = d->nsite;
+  fill_params(d, g->prm);
+  g->nt = d->block_threads ? d->block_threads : 256;
+  // window fraction along the sort axis ~ 2 (rcut + margin) / L
+  {
+    const int axis = g->prm.sort_axis;
+    double wf = std::min(1.0, 2.0 * (d->rcut + d->margin) / sc[0].L[axis]);
+    g->prm.chunks = d->chunks ? d->chunks : pick_chunks(g->N, g->nt, wf);
+  }
+  g->lds = lds_bytes_host(g->N, g->NS, g->prm.precision, g->nt);
+  const int lim = device_lds_limit();
+  if (g->lds > lim) {
+    free_job(g);
+    return fail(REMD_EINVAL, "replica too large for workgroup LDS (" + std::to_string(g->lds) + " > " +
+                                 std::to_string(lim) + " bytes)");
+  }
+#define CK(...)                        \
+  do {                                 \
+    int rc_ = (__VA_ARGS__);           \
+    if (rc_) {                         \
+      std::string m_ = g_err;          \
+      free_job(g);                     \
+      return fail(rc_, m_);            \
+    }                                  \
+  } while (0)
+  CK([&]() -> int {
+    HIPCHK(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&g->ev0));
+    HIPCHK(hipEventCreate(&g->ev1));
+    const size_t N = g->N, R = g->nrep;
+    HIPCHK(hipMalloc(&g->d_job, sizeof(JobParams)));
+    HIPCHK(hipMalloc(&g->d_mol, sizeof(double) * NFIELD * N * R));
+    HIPCHK(hipMalloc(&g->d_mid, sizeof(int) * N * R));
+    HIPCHK(hipMalloc(&g->d_sc, sizeof(double) * NSC * R));
+    HIPCHK(hipMalloc(&g->d_reps, sizeof(ReplicaDesc) * R));
+    HIPCHK(hipHostMalloc(&g->h_sc, sizeof(double) * NSC * R, hipHostMallocDefault));
+    return 0;
+  }());
+  CK(build_wall(g));
+  g->prm.fwall = g->d_fwall;
+  CK([&]() -> int {
+    HIPCHK(hipMemcpyAsync(g->d_job, &g->prm, sizeof(JobParams), hipMemcpyHostToDevice, g->stream));
+    const size_t N = g->N, R = g->nrep;
+    std::vector<double> mol(NFIELD * N * R, 0.0);
+    std::vector<int> mid(N * R);
+    for (size_t rr = 0; rr < R; rr++) {
+      double *m = mol.data() + rr * NFIELD * N;
+      for (size_t i = 0; i < N; i++) {
+        const size_t k = rr * N + i;
+        for (int dd = 0; dd < 3; dd++) {
+          m[(F_RX + dd) * N + i] = r[3 * k + dd];
+          m[(F_VX + dd) * N + i] = v[3 * k + dd];
+        }
+        for (int dd = 0; dd < 4; dd++) {
+          m[(F_Q0 + dd) * N + i] = q[4 * k + dd];
+          m[(F_P0 + dd) * N + i] = p[4 * k + dd];
+        }
+        mid[k] = ids ? ids[k] : (int)i;
+      }
+    }
+    HIPCHK(hipMemcpyAsync(g->d_mol, mol.data(), sizeof(double) * mol.size(), hipMemcpyHostToDevice, g->stream));
+    HIPCHK(hipMemcpyAsync(g->d_mid, mid.data(), sizeof(int) * mid.size(), hipMemcpyHostToDevice, g->stream));
+    for (size_t rr = 0; rr < R; rr++) {
+      double *s = g->h_sc + rr * NSC;
+      memset(s, 0, sizeof(double) * NSC);
+      s[S_S] = sc[rr].s;
+      s[S_PS] = sc[rr].Ps;
+      s[S_Q] = sc[rr].Q;
+      for (int dd = 0; dd < 3; dd++) s[S_PV0 + dd] = sc[rr].Pv[dd];
+      s[S_W] = sc[rr].W;
+      for (int dd = 0; dd < 3; dd++) s[S_L0 + dd] = sc[rr].L[dd];
+      s[S_T] = s[S_TNEW] = sc[rr].T;
+      s[S_P] = s[S_PNEW] = sc[rr].P;
+    }
+    HIPCHK(hipMemcpyAsync(g->d_sc, g->h_sc, sizeof(double) * NSC * R, hipMemcpyHostToDevice, g->stream));
+    int rc2 = upload_reps(g);
+    if (rc2) return rc2;
+    // the staging vectors above are pageable and go out of scope here
+    HIPCHK(hipStreamSynchronize(g->stream));
+    return 0;
+  }());
+  CK(launch(g, MODE_INIT, 1, 0.0, g->d_reps, g->nrep, false));
+  CK(sync_scalars(g));
+#undef CK
+  *out = g;
+  return 0;
+}
+
+extern "C" void remd_gpu_destroy(remd_gpu *g) {
+  if (g && g->stream) hipStreamSynchronize(g->stream);
+  free_job(g);
+}
+
+extern "C" int remd_gpu_set_conditions(remd_gpu *g, const double *T, const double *P) {
+  if (!g || !T || !P) return fail(REMD_EINVAL, "null argument");
+  for (int r = 0; r < g->nrep; r++) {
+    double *s = g->h_sc + (size_t)r * NSC;
+    if (!(T[r] > 0)) return fail(REMD_EINVAL, "T must be > 0");
+    const bool changed = (T[r] != s[S_T]) || (P[r] != s[S_P]);
+    s[S_TNEW] = T[r];
+    s[S_PNEW] = P[r];
+    s[S_XFLAG] = changed ? 1.0 : 0.0;
+  }
+  HIPCHK(hipMemcpyAsync(g->d_sc, g->h_sc, sizeof(double) * NSC * g->nrep, hipMemcpyHostToDevice, g->stream));
+  return 0;
+}
+
+extern "C" int remd_gpu_execute_steps(remd_gpu *g, const double *T, const double *P) {
+  int rc = remd_gpu_set_conditions(g, T, P);
+  if (rc) return rc;
+  rc = launch(g, MODE_INTERVAL, 0, 0.0, g->d_reps, g->nrep, true);
+  if (rc) return rc;
+  rc = sync_scalars(g);
+  if (rc) return rc;
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, g->ev0, g->ev1));
+  g->last_ms = ms;
+  return 0;
+}
+
+extern "C" int remd_gpu_execute_pending(remd_gpu *g) {
+  if (!g) return fail(REMD_EINVAL, "null handle");
+  int rc = launch(g, MODE_INTERVAL, 0, 0.0, g->d_reps, g->nrep, true);
+  if (rc) return rc;
+  rc = sync_scalars(g);
+  if (rc) return rc;
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, g->ev0, g->ev1));
+  g->last_ms = ms;
+  return 0;
+}
+
+extern "C" int remd_gpu_execute_steps_plain(remd_gpu *g, const double *T, const double *P) {
+  int rc = remd_gpu_set_conditions(g, T, P);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(g->ev0, g->stream));
+  for (int r = 0; r < g->nrep; r++) {
+    LaunchArgs a;
+    a.reps = g->d_reps + r;
+    a.nrep = 1;
+    a.mode = MODE_INTERVAL;
+    a.do_sort = 0;
+    a.margin = 0.0;
+    HIPCHK(launch_interval(a, g->nt, g->NS, g->prm.precision, g->lds, g->stream));
+  }
+  HIPCHK(hipEventRecord(g->ev1, g->stream));
+  rc = sync_scalars(g);
+  if (rc) return rc;
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, g->ev0, g->ev1));
+  g->last_ms = ms;
+  return 0;
+}
+
+extern "C" int remd_gpu_compute_forces(remd_gpu *g, int do_sort, double margin) {
+  if (!g) return fail(REMD_EINVAL, "null handle");
+  if (!do_sort && !g->debug) return fail(REMD_ESTATE, "reusing the window needs remd_gpu_set_debug(g, 1)");
+  int rc = launch(g, MODE_FORCE, do_sort, margin, g->d_reps, g->nrep, true);
+  if (rc) return rc;
+  rc = sync_scalars(g);
+  if (rc) return rc;
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, g->ev0, g->ev1));
+  g->last_ms = ms;
+  return 0;
+}
+
+extern "C" int remd_gpu_set_debug(remd_gpu *g, int on) {
+  if (!g) return fail(REMD_EINVAL, "null handle");
+  if (on && !g->d_site) {
+    HIPCHK(hipMalloc(&g->d_site, sizeof(double) * g->nrep * g->N * g->NS * 3));
+    HIPCHK(hipMalloc(&g->d_win, sizeof(int) * g->nrep * g->N * 2));
+    HIPCHK(hipMemsetAsync(g->d_site, 0, sizeof(double) * g->nrep * g->N * g->NS * 3, g->stream));
+    HIPCHK(hipMemsetAsync(g->d_win, 0, sizeof(int) * g->nrep * g->N * 2, g->stream));
+  }
+  g->debug = on != 0;
+  int rc = upload_reps(g);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+extern "C" int remd_gpu_get_site_forces(remd_gpu *g, int rep, double *af) {
+  if (!g || !af || rep < 0 || rep >= g->nrep) return fail(REMD_EINVAL, "bad argument");
+  if (!g->d_site) return fail(REMD_ESTATE, "debug buffers are off");
+  HIPCHK(hipMemcpy(af, g->d_site + (size_t)rep * g->N * g->NS * 3, sizeof(double) * g->N * g->NS * 3,
+                   hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int remd_gpu_get_window(remd_gpu *g, int rep, int *js, int *je) {
+  if (!g || !js || !je || rep < 0 || rep >= g->nrep) return fail(REMD_EINVAL, "bad argument");
+  if (!g->d_win) return fail(REMD_ESTATE, "debug buffers are off");
+  std::vector<int> w(2 * g->N);
+  HIPCHK(hipMemcpy(w.data(), g->d_win + (size_t)rep * g->N * 2, sizeof(int) * 2 * g->N, hipMemcpyDeviceToHost));
+  for (int i = 0; i < g->N; i++) {
+    js[i] = w[2 * i];
+    je[i] = w[2 * i + 1];
+  }
+  return 0;
+}
+
+extern "C" int remd_gpu_last_kernel_ms(remd_gpu *g, double *ms) {
+  if (!g || !ms) return fail(REMD_EINVAL, "null argument");
+  *ms = g->last_ms;
+  return 0;
+}
+
+extern "C" int remd_gpu_launch_shape(remd_gpu *g, int *bt, int *ch, int *lds) {
+  if (!g) return fail(REMD_EINVAL, "null handle");
+  if (bt) *bt = g->nt;
+  if (ch) *ch = g->prm.chunks;
+  if (lds) *lds = g->lds;
+  return 0;
+}
+
+extern "C" int remd_gpu_autotune(remd_gpu *g, int reps) {
+  if (!g) return fail(REMD_EINVAL, "null handle");
+  if (reps < 1) reps = 3;
+  const int nts[2] = {128, 256};
+  double best = 1e300;
+  int bnt = g->nt, bc = g->prm.chunks;
+  const int lim = device_lds_limit();
+  for (int a = 0; a < 2; a++) {
+    for (int C = 1; C <= 16; C *= 2) {
+      const int lds = lds_bytes_host(g->N, g->NS, g->prm.precision, nts[a]);
+      if (lds > lim) continue;
+      g->nt = nts[a];
+      g->lds = lds;
+      g->prm.chunks = C;
+      HIPCHK(hipMemcpyAsync(g->d_job, &g->prm, sizeof(JobParams), hipMemcpyHostToDevice, g->stream));
+      double tsum = 0;
+      for (int k = 0; k <= reps; k++) {
+        int rc = launch(g, MODE_FORCE, 1, g->prm.margin, g->d_reps, g->nrep, true);
+        if (rc) return rc;
+        HIPCHK(hipEventSynchronize(g->ev1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, g->ev0, g->ev1));
+        if (k > 0) tsum += ms;  // first launch is a warm-up
+      }
+      if (tsum < best) {
+        best = tsum;
+        bnt = nts[a];
+        bc = C;
+      }
+    }
+  }
+  g->nt = bnt;
+  g->prm.chunks = bc;
+  g->lds = lds_bytes_host(g->N, g->NS, g->prm.precision, bnt);
+  HIPCHK(hipMemcpyAsync(g->d_job, &g->prm, sizeof(JobParams), hipMemcpyHostToDevice, g->stream));
+  // restore the force/energies of the resident state for the chosen shape
+  int rc = launch(g, MODE_FORCE, 1, g->prm.margin, g->d_reps, g->nrep, false);
+  if (rc) return rc;
+  return sync_scalars(g);
+}
+
+extern "C" int remd_gpu_get_averages(remd_gpu *g, int rep, double out[19]) {
+  if (!g || !out || rep < 0 || rep >= g->nrep) return fail(REMD_EINVAL, "bad argument");
+  const double *s = g->h_sc + (size_t)rep * NSC;
+  out[0] = s[S_LJ];
+  out[1] = s[S_CLMB];
+  out[2] = s[S_WALL];
+  out[3] = s[S_VOL];
+  out[4] = s[S_TRA];
+  out[5] = s[S_ROT];
+  out[6] = s[S_TK];
+  out[7] = s[S_PRS3];
+  out[8] = s[S_PRS0];
+  out[9] = s[S_PRS1];
+  out[10] = s[S_PRS2];
+  out[11] = s[S_TOT];
+  out[12] = s[S_HAM];
+  out[13] = s[S_S];
+  out[14] = s[S_PS];
+  out[15] = s[S_PV0];
+  out[16] = s[S_PV1];
+  out[17] = s[S_PV2];
+  out[18] = 0.0;
+  return 0;
+}
+
+extern "C" int remd_gpu_get_pot_vol(remd_gpu *g, double *pot, double *vol) {
+  if (!g) return fail(REMD_EINVAL, "null handle");
+  for (int r = 0; r < g->nrep; r++) {
+    const double *s = g->h_sc + (size_t)r * NSC;
+    if (pot) pot[r] = s[S_LJ] + s[S_CLMB] + s[S_WALL];
+    if (vol) vol[r] = s[S_VOL];
+  }
+  return 0;
+}
+
+extern "C" int remd_gpu_download_state(remd_gpu *g, int rep, double *r, double *v, double *f, double *q,
+                                       double *p, double *n, int *ids, remd_gpu_scalars *sc) {
+  if (!g || rep < 0 || rep >= g->nrep) return fail(REMD_EINVAL, "bad argument");
+  const size_t N = g->N;
+  std::vector<double> m(NFIELD * N);
+  HIPCHK(hipMemcpyAsync(m.data(), g->d_mol + (size_t)rep * NFIELD * N, sizeof(double) * NFIELD * N,
+                        hipMemcpyDeviceToHost, g->stream));
+  std::vector<int> mid(N);
+  HIPCHK(hipMemcpyAsync(mid.data(), g->d_mid + (size_t)rep * N, sizeof(int) * N, hipMemcpyDeviceToHost,
+                        g->stream));
+  HIPCHK(hipStreamSynchronize(g->stream));
+  for (size_t i = 0; i < N; i++) {
+    for (int d = 0; d < 3; d++) {
+      if (r) r[3 * i + d] = m[(F_RX + d) * N + i];
+      if (v) v[3 * i + d] = m[(F_VX + d) * N + i];
+      if (f) f[3 * i + d] = m[(F_FX + d) * N + i];
+    }
+    for (int d = 0; d < 4; d++) {
+      if (q) q[4 * i + d] = m[(F_Q0 + d) * N + i];
+      if (p) p[4 * i + d] = m[(F_P0 + d) * N + i];
+      if (n) n[4 * i + d] = m[(F_N0 + d) * N + i];
+    }
+    if (ids) ids[i] = mid[i];
+  }
+  if (sc) {
+    const double *s = g->h_sc + (size_t)rep * NSC;
+    sc->s = s[S_S];
+    sc->Ps = s[S_PS];
+    sc->Q = s[S_Q];
+    for (int d = 0; d < 3; d++) sc->Pv[d] = s[S_PV0 + d];
+    sc->W = s[S_W];
+    for (int d = 0; d < 3; d++) sc->L[d] = s[S_L0 + d];
+    sc->T = s[S_T];
+    sc->P = s[S_P];
+  }
+  return 0;
+}
+
+// body_to_space (src/rotation.h:29-48)
+static void b2s(const double *a, const double *v, double *t) {
+  const double a0sq = a[0] * a[0], a1sq = a[1] * a[1], a2sq = a[2] * a[2], a3sq = a[3] * a[3];
+  const double a01 = a[0] * a[1], a02 = a[0] * a[2], a03 = a[0] * a[3];
+  const double a12 = a[1] * a[2], a13 = a[1] * a[3], a23 = a[2] * a[3];
+  t[0] = (a0sq + a1sq - a2sq - a3sq) * v[0] + 2. * (a12 - a03) * v[1] + 2. * (a13 + a02) * v[2];
+  t[1] = 2. * (a12 + a03) * v[0] + (a0sq - a1sq + a2sq - a3sq) * v[1] + 2. * (a23 - a01) * v[2];
+  t[2] = 2. * (a13 - a02) * v[0] + 2. * (a23 + a01) * v[1] + (a0sq - a1sq - a2sq + a3sq) * v[2];
+}
+
+extern "C" int remd_gpu_write_cdv(remd_gpu *g, const char *prefix, unsigned step) {
+  if (!g || !prefix) return fail(REMD_EINVAL, "null argument");
+  const size_t N = g->N;
+  std::vector<double> r(3 * N), q(4 * N);
+  for (int rep = 0; rep < g->nrep; rep++) {
+    remd_gpu_scalars sc;
+    int rc = remd_gpu_download_state(g, rep, r.data(), nullptr, nullptr, q.data(), nullptr, nullptr, nullptr, &sc);
+    if (rc) return rc;
+    char name[4096];
+    snprintf(name, sizeof(name), "%sR%04ds%07u.cdv", prefix, rep, step);
+    FILE *fp = fopen(name, "w");
+    if (!fp) return fail(REMD_EINVAL, std::string("cannot open ") + name);
+    const double *L = sc.L;
+    const double R = g->desc.wall_length;
+    if (g->desc.confined == 1)
+      fprintf(fp, "' box_sx=%g box_sy=%g box_sz=%g box_ex=%g box_ey=%g box_ez=%g", -R, -R, -0.5 * L[2], R, R,
+              0.5 * L[2]);
+    else if (g->desc.confined == 2)
+      fprintf(fp, "' box_sx=%g box_sy=%g box_sz=%g box_ex=%g box_ey=%g box_ez=%g", -0.5 * L[0], -0.5 * L[1],
+              -0.5 * R, 0.5 * L[0], 0.5 * L[1], 0.5 * R);
+    else
+      fprintf(fp, "' box_sx=%g box_sy=%g box_sz=%g box_ex=%g box_ey=%g box_ez=%g", -0.5 * L[0], -0.5 * L[1],
+              -0.5 * L[2], 0.5 * L[0], 0.5 * L[1], 0.5 * L[2]);
+    fprintf(fp, " r0=1.0 r1=0.8 r2=0.8 c0=(1.0,0.0,0.0) c1=(1.0,1.0,1.0) c2=(1.0,1.0,1.0)"
+                " bond0_c=(1.0,1.0,1.0) st0=\"T=%g K, P=%g MPa\" st0_pos=(-2.0,-2.0)\n",
+            sc.T * unit_temp(), sc.P * unit_press() / 1e6);
+    for (size_t i = 0; i < N; i++) {
+      for (int a = 0; a < g->NS; a++) {
+        double t[3];
+        b2s(&q[4 * i], g->prm.site_r[a], t);
+        fprintf(fp, "%zu %d %g %g %g\n", i, a, (r[3 * i] - 0.5) * L[0] + t[0], (r[3 * i + 1] - 0.5) * L[1] + t[1],
+                (r[3 * i + 2] - 0.5) * L[2] + t[2]);
+      }
+    }
+    fclose(fp);
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------- wall table
+extern "C" int remd_gpu_wall_table(int confined, int nfwall, double R, double sgm, double eps, double rho,
+                                   double *out) {
+  if (!out || nfwall < 3 || !(R > 0) || (confined != 1 && confined != 2))
+    return fail(REMD_EINVAL, "bad wall table arguments");
+  double *d = nullptr;
+  HIPCHK(hipMalloc(&d, sizeof(double) * 2 * nfwall));
+  hipError_t e;
+  if (confined == 1) {
+    const double sgm06 = (double)powf((float)sgm, 6.0f);
+    e = launch_wall_tube(nfwall, R, sgm06, M_PI * rho * eps * sgm06, d, nullptr);
+  } else {
+    e = launch_wall_slit(nfwall, R, sgm, eps, rho, d, nullptr);
+  }
+  if (e == hipSuccess) e = hipMemcpy(out, d, sizeof(double) * 2 * nfwall, hipMemcpyDeviceToHost);
+  hipFree(d);
+  if (e != hipSuccess) return fail(REMD_EHIP, std::string("wall table: ") + hipGetErrorString(e));
+  return 0;
+}
+
+// --------------------------------------------------------- sort / window
+extern "C" int remd_gpu_sort_window(const double *keys, int nrow, int n, double w, int *order, int *js,
+                                    int *je) {
+  if (!keys || !order || !js || !je || nrow < 1 || n < 2 || n > 4096)
+    return fail(REMD_EINVAL, "bad sort arguments");
+  const int lds = 20 * n + 8 + (16 * 64 + 16) * 4;
+  if (lds > device_lds_limit()) return fail(REMD_EINVAL, "row too long for LDS");
+  double *dk = nullptr;
+  int *dout = nullptr;
+  HIPCHK(hipMalloc(&dk, sizeof(double) * (size_t)nrow * n));
+  hipError_t e = hipMalloc(&dout, sizeof(int) * 3 * (size_t)nrow * n);
+  if (e == hipSuccess) e = hipMemcpy(dk, keys, sizeof(double) * (size_t)nrow * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = launch_sort_window(dk, nrow, n, w, dout, dout + (size_t)nrow * n, dout + 2 * (size_t)nrow * n, nullptr);
+  if (e == hipSuccess) e = hipMemcpy(order, dout, sizeof(int) * (size_t)nrow * n, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(js, dout + (size_t)nrow * n, sizeof(int) * (size_t)nrow * n, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(je, dout + 2 * (size_t)nrow * n, sizeof(int) * (size_t)nrow * n, hipMemcpyDeviceToHost);
+  hipFree(dk);
+  hipFree(dout);
+  if (e != hipSuccess) return fail(REMD_EHIP, std::string("sort_window: ") + hipGetErrorString(e));
+  return 0;
+}
+
+// -------------------------------------------------------------- exchange
+static int pair_count(int dt, int dp, int type) {
+  if (type == 0) return dp * (dt / 2);
+  if (type == 1) return (dp / 2) * dt;
+  if (type == 2) return dp * std::max(0, dt / 2 - (dt + 1) % 2);
+  return std::max(0, dp / 2 - (dp + 1) % 2) * dt;
+}
+
+extern "C" int remd_gpu_exchange_device(remd_gpu *g, int ntemp, int npress, long step, int *index, double *T,
+                                        double *P, const double *u, int *accepted, int *npairs) {
+  if (!g || !index || !T || !P || !u) return fail(REMD_EINVAL, "null argument");
+  if (ntemp * npress != g->nrep) return fail(REMD_EINVAL, "ntemp*npress must equal nreplica");
+  const int R = g->nrep;
+  {
+    // index must be a permutation of the replicas: the kernel dereferences it
+    std::vector<char> seen(R, 0);
+    for (int k = 0; k < R; k++) {
+      if (index[k] < 0 || index[k] >= R || seen[index[k]]) return fail(REMD_EINVAL, "index is not a permutation");
+      seen[index[k]] = 1;
+      if (!(T[k] > 0)) return fail(REMD_EINVAL, "T must be > 0");
+    }
+  }
+  const int ec = (int)(step % 4);
+  const int np = pair_count(ntemp, npress, ec);
+  if (!g->d_index) {
+    HIPCHK(hipMalloc(&g->d_index, sizeof(int) * R));
+    HIPCHK(hipMalloc(&g->d_T, sizeof(double) * R));
+    HIPCHK(hipMalloc(&g->d_P, sizeof(double) * R));
+    HIPCHK(hipMalloc(&g->d_u, sizeof(double) * (R + 1)));
+    HIPCHK(hipMalloc(&g->d_acc, sizeof(int) * (R + 1)));
+    HIPCHK(hipMalloc(&g->d_scptr, sizeof(double *) * R));
+    HIPCHK(hipMalloc(&g->d_xjob, sizeof(ExchangeJob)));
+    std::vector<double *> ptrs(R);
+    for (int r = 0; r < R; r++) ptrs[r] = g->d_sc + (size_t)r * NSC;
+    HIPCHK(hipMemcpy(g->d_scptr, ptrs.data(), sizeof(double *) * R, hipMemcpyHostToDevice));
+  }
+  ExchangeJob X;
+  X.ntemp = ntemp;
+  X.npress = npress;
+  X.pmode = g->desc.pconstant;
+  X.ec_type = ec;
+  X.index = g->d_index;
+  X.T = g->d_T;
+  X.P = g->d_P;
+  X.u = g->d_u;
+  X.sc = g->d_scptr;
+  X.accepted = g->d_acc;
+  HIPCHK(hipMemcpyAsync(g->d_xjob, &X, sizeof(X), hipMemcpyHostToDevice, g->stream));
+  HIPCHK(hipMemcpyAsync(g->d_index, index, sizeof(int) * R, hipMemcpyHostToDevice, g->stream));
+  HIPCHK(hipMemcpyAsync(g->d_T, T, sizeof(double) * R, hipMemcpyHostToDevice, g->stream));
+  HIPCHK(hipMemcpyAsync(g->d_P, P, sizeof(double) * R, hipMemcpyHostToDevice, g->stream));
+  if (np > 0) HIPCHK(hipMemcpyAsync(g->d_u, u, sizeof(double) * np, hipMemcpyHostToDevice, g->stream));
+  HIPCHK(launch_exchange(g->d_xjob, 1, g->stream));
+  HIPCHK(hipMemcpyAsync(index, g->d_index, sizeof(int) * R, hipMemcpyDeviceToHost, g->stream));
+  HIPCHK(hipMemcpyAsync(T, g->d_T, sizeof(double) * R, hipMemcpyDeviceToHost, g->stream));
+  HIPCHK(hipMemcpyAsync(P, g->d_P, sizeof(double) * R, hipMemcpyDeviceToHost, g->stream));
+  std::vector<int> acc(np > 0 ? np : 1);
+  if (np > 0) HIPCHK(hipMemcpyAsync(acc.data(), g->d_acc, sizeof(int) * np, hipMemcpyDeviceToHost, g->stream));
+  HIPCHK(hipStreamSynchronize(g->stream));
+  if (accepted)
+    for (int k = 0; k < np; k++) accepted[k] = acc[k];
+  if (npairs) *npairs = np;
+  // keep the host mirror's conditions in step with the device
+  return sync_scalars(g);
+}
+
+// ----------------------------------------------------------------- batch
+extern "C" int remd_batch_create(remd_batch **out) {
+  if (!out) return fail(REMD_EINVAL, "null output");
+  remd_batch *b = new remd_batch;
+  hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+  for (int c = 0; c < NCLASS && e == hipSuccess; c++) {
+    e = hipStreamCreateWithFlags(&b->cstream[c], hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->cdone[c], hipEventDisableTiming);
+  }
+  if (e != hipSuccess) {
+    delete b;
+    return fail(REMD_EHIP, hipGetErrorString(e));
+  }
+  *out = b;
+  return 0;
+}
+
+extern "C" int remd_batch_add(remd_batch *b, remd_gpu *g) {
+  if (!b || !g) return fail(REMD_EINVAL, "null argument");
+  if (std::find(b->jobs.begin(), b->jobs.end(), g) != b->jobs.end()) return 0;
+  b->jobs.push_back(g);
+  b->dirty = true;
+  return 0;
+}
+
+extern "C" int remd_batch_remove(remd_batch *b, remd_gpu *g) {
+  if (!b || !g) return fail(REMD_EINVAL, "null argument");
+  auto it = std::find(b->jobs.begin(), b->jobs.end(), g);
+  if (it == b->jobs.end()) return fail(REMD_EINVAL, "job not in batch");
+  b->jobs.erase(it);
+  b->dirty = true;
+  return 0;
+}
+
+// Table of every replica of the batch grouped by kernel class, heaviest
+// first within a class (workgroups are dispatched in blockIdx order, so long
+// replicas start early).
+static int rebuild_table(remd_batch *b) {
+  std::vector<std::pair<std::pair<int, double>, ReplicaDesc>> rows;
+  for (int c = 0; c < NCLASS; c++) b->cls_lds[c] = b->cls_n[c] = b->cls_off[c] = 0;
+  for (remd_gpu *g : b->jobs) {
+    refresh_reps(g);
+    const int c = kclass(g->NS, g->prm.precision);
+    const double w = (double)g->N * g->N * g->NS * g->NS * std::max(1, g->desc.interval);
+    for (int r = 0; r < g->nrep; r++) rows.push_back({{c, -w}, g->h_reps[r]});
+    b->cls_lds[c] = std::max(b->cls_lds[c], lds_bytes_host(g->N, g->NS, g->prm.precision, b->nt));
+    b->cls_n[c] += g->nrep;
+  }
+  std::stable_sort(rows.begin(), rows.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+  int off = 0;
+  for (int c = 0; c < NCLASS; c++) {
+    b->cls_off[c] = off;
+    off += b->cls_n[c];
+  }
+  const int n = (int)rows.size();
+  if (n > b->table_cap) {
+    hipFree(b->d_table);
+    b->d_table = nullptr;
+    HIPCHK(hipMalloc(&b->d_table, sizeof(ReplicaDesc) * std::max(n, 1)));
+    b->table_cap = n;
+  }
+  std::vector<ReplicaDesc> flat(n);
+  for (int k = 0; k < n; k++) flat[k] = rows[k].second;
+  if (n) HIPCHK(hipMemcpyAsync(b->d_table, flat.data(), sizeof(ReplicaDesc) * n, hipMemcpyHostToDevice, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  b->ntab = n;
+  b->dirty = false;
+  return 0;
+}
+
+extern "C" int remd_batch_execute(remd_batch *b) {
+  if (!b) return fail(REMD_EINVAL, "null batch");
+  if (b->jobs.empty()) return 0;
+  for (remd_gpu *g : b->jobs)
+    if (g->nt != b->nt) return fail(REMD_EINVAL, "batched jobs use 256-thread workgroups");
+  if (b->dirty) {
+    int rc = rebuild_table(b);
+    if (rc) return rc;
+  }
+  // conditions were staged on each job's stream; order them before the launch
+  for (remd_gpu *g : b->jobs) HIPCHK(hipStreamSynchronize(g->stream));
+  HIPCHK(hipEventRecord(b->ev0, b->stream));
+  for (int c = 0; c < NCLASS; c++) {
+    if (!b->cls_n[c]) continue;
+    if (b->cls_lds[c] > device_lds_limit()) return fail(REMD_EINVAL, "batch LDS exceeds the device limit");
+    HIPCHK(hipStreamWaitEvent(b->cstream[c], b->ev0, 0));
+    LaunchArgs a;
+    a.reps = b->d_table + b->cls_off[c];
+    a.nrep = b->cls_n[c];
+    a.mode = MODE_INTERVAL;
+    a.do_sort = 0;
+    a.margin = 0.0;
+    HIPCHK(launch_interval(a, b->nt, 3 + c / 2, c % 2, b->cls_lds[c], b->cstream[c]));
+    HIPCHK(hipEventRecord(b->cdone[c], b->cstream[c]));
+    HIPCHK(hipStreamWaitEvent(b->stream, b->cdone[c], 0));
+  }
+  HIPCHK(hipEventRecord(b->ev1, b->stream));
+  for (remd_gpu *g : b->jobs)
+    HIPCHK(hipMemcpyAsync(g->h_sc, g->d_sc, sizeof(double) * NSC * g->nrep, hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  b->last_ms = ms;
+  return 0;
+}
+
+extern "C" int remd_batch_last_kernel_ms(remd_batch *b, double *ms) {
+  if (!b || !ms) return fail(REMD_EINVAL, "null argument");
+  *ms = b->last_ms;
+  return 0;
+}
+
+extern "C" void remd_batch_destroy(remd_batch *b) {
+  if (!b) return;
+  if (b->stream) hipStreamSynchronize(b->stream);
+  hipFree(b->d_table);
+  for (int c = 0; c < NCLASS; c++) {
+    if (b->cdone[c]) hipEventDestroy(b->cdone[c]);
+    if (b->cstream[c]) hipStreamDestroy(b->cstream[c]);
+  }
+  if (b->ev0) hipEventDestroy(b->ev0);
+  if (b->ev1) hipEventDestroy(b->ev1);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+}
+
+// ---------------------------------------------------------------- rng
+// glibc random() TYPE_3 (the generator behind rand()): additive lagged
+// Fibonacci r[i] = r[i-3] + r[i-31] (mod 2^32), seeded with the
+// 16807 mod (2^31-1) Lehmer sequence, first 310 outputs discarded, each
+// output r[i] >> 1 (as described by P. Selinger, "The GLIBC pseudo-random
+// number generator").  State: the last 34 words, k = next index mod 34.
+extern "C" void remd_rng_seed(remd_rng *s, unsigned seed) {
+  if (seed == 0) seed = 1;
+  int32_t r[344];
+  r[0] = (int32_t)seed;
+  for (int i = 1; i < 31; i++) {
+    long long w = (16807LL * r[i - 1]) % 2147483647LL;
+    if (w < 0) w += 2147483647LL;
+    r[i] = (int32_t)w;
+  }
+  for (int i = 31; i < 34; i++) r[i] = r[i - 31];
+  for (int i = 34; i < 344; i++) r[i] = (int32_t)((uint32_t)r[i - 31] + (uint32_t)r[i - 3]);
+  for (int i = 0; i < 34; i++) s->r[i] = r[310 + i];
+  s->k = 0;  // next index 344 == 310 + 34  ->  slot 0 of the ring
+}
+
+extern "C" int remd_rng_next(remd_rng *s) {
+  const int k = s->k;
+  const uint32_t v = (uint32_t)s->r[(k + 34 - 31) % 34] + (uint32_t)s->r[(k + 34 - 3) % 34];
+  s->r[k] = (int32_t)v;
+  s->k = (k + 1) % 34;
+  return (int)(v >> 1);
+}

@@ -1,0 +1,1345 @@
+// remd_kernels.hip -- the REMD per-replica hot path for gfx950 (MI355X).
+//
+// One persistent workgroup per replica runs a whole exchange interval:
+//   Nose-Poincare thermostat + Andersen barostat rigid-body splitting
+//   D6 D5 D4 D3 D2 D1 D2 D3 (PsPv) D4 | force | D5 D6
+//   (src/integrator.cxx:646-706, the CPU canonical order), with
+//   - the z-sorted neighbour window rebuilt every sort_interval steps by a
+//     wave-ballot LSD radix sort in LDS (replaces src/sort.cu +
+//     src/quicksort.cu; window semantics of src/molecules.cxx:541-585),
+//   - M2A into LDS site records (FP32 centred COM + FP32 site offsets,
+//     rotated in FP64; src/molecules.cxx:319-360, src/rotation.h:29-48),
+//   - the switched LJ + Coulomb pair pass (src/force_calculator.cxx:803-989
+//     semantics) with FP32 pair math and FP64 accumulation, work items
+//     (i, chunk) mapped to adjacent lanes and reduced by a fixed xor
+//     butterfly (deterministic, independent of what else is resident),
+//   - the wall table lookup (src/force_calculator.cxx:1494-1531) and A2M
+//     torque (src/molecules.cxx:401-439) in FP64,
+//   - block reductions in FP64 with a fixed tree.
+// All O(N) integrator state is FP64; only the O(N * neighbours) pair math is
+// FP32 (precision == 1 switches it to FP64 for the precision study).
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#include "remd_common.h"
+
+namespace remd {
+
+constexpr int WAVE = 64;
+
+// ------------------------------------------------------------ LDS layout
+template <int NS>
+struct Rec {
+  // COM (3) + NS sites x 3, padded for 8/16-byte LDS reads
+  static constexpr int STRIDE = (NS == 5) ? 18 : (NS == 4 ? 16 : 12);
+};
+
+__host__ __device__ inline int align16(int x) { return (x + 15) & ~15; }
+
+// Bytes of dynamic LDS a replica of nmol molecules needs.
+__host__ __device__ inline int lds_bytes(int nmol, int nsite, int prec, int nt) {
+  const int stride = (nsite == 5) ? 18 : (nsite == 4 ? 16 : 12);
+  const int rec = nmol * stride * (prec ? 8 : 4);
+  const int sortb = 20 * nmol;
+  int region0 = align16(rec > sortb ? rec : sortb);
+  const int win = align16(4 * nmol);
+  const int misc = (nt / WAVE) * 32 * 8 + (16 * 64 + 16) * 4;
+  return region0 + win + align16(misc);
+}
+
+struct Lds {
+  char *region0;
+  uint16_t *js, *je;
+  double *red;   // NW x 32 doubles
+  int *cnt;      // radix counts
+};
+
+template <int NT>
+__device__ inline Lds lds_carve(char *smem, int nmol, int nsite, int prec) {
+  const int stride = (nsite == 5) ? 18 : (nsite == 4 ? 16 : 12);
+  const int rec = nmol * stride * (prec ? 8 : 4);
+  const int sortb = 20 * nmol;
+  const int region0 = align16(rec > sortb ? rec : sortb);
+  Lds L;
+  L.region0 = smem;
+  L.js = (uint16_t *)(smem + region0);
+  L.je = L.js + nmol;
+  char *misc = smem + region0 + align16(4 * nmol);
+  L.red = (double *)misc;
+  L.cnt = (int *)(misc + (NT / WAVE) * 32 * 8);
+  return L;
+}
+
+// ------------------------------------------------------------ reductions
+__device__ inline double wave_sum(double v) {
+  // xor butterfly: a+b == b+a exactly, so every lane ends with the same bits
+#pragma unroll
+  for (int m = 1; m < WAVE; m <<= 1) v += __shfl_xor(v, m, WAVE);
+  return v;
+}
+
+// Sum K doubles over the workgroup; every thread receives the same result.
+template <int NT, int K>
+__device__ inline void block_sum(double (&v)[K], double *red) {
+  constexpr int NW = NT / WAVE;
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int w = threadIdx.x / WAVE;
+#pragma unroll
+  for (int k = 0; k < K; k++) v[k] = wave_sum(v[k]);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) red[w * K + k] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    double s = 0.0;
+    for (int ww = 0; ww < NW; ww++) s += red[ww * K + k];
+    v[k] = s;
+  }
+  __syncthreads();
+}
+
+// ------------------------------------------------------- replica scalars
+struct Sc {
+  double s, Ps, Q, Pv[3], W, L[3], T, P, gkT, H0;
+  double lj, clmb, wall, pot, vir[3];
+};
+
+__device__ inline void load_sc(Sc &c, const double *sc) {
+  c.s = sc[S_S]; c.Ps = sc[S_PS]; c.Q = sc[S_Q];
+  c.Pv[0] = sc[S_PV0]; c.Pv[1] = sc[S_PV1]; c.Pv[2] = sc[S_PV2]; c.W = sc[S_W];
+  c.L[0] = sc[S_L0]; c.L[1] = sc[S_L1]; c.L[2] = sc[S_L2];
+  c.T = sc[S_T]; c.P = sc[S_P]; c.gkT = sc[S_GKT]; c.H0 = sc[S_H0];
+  c.lj = sc[S_LJ]; c.clmb = sc[S_CLMB]; c.wall = sc[S_WALL]; c.pot = sc[S_POT];
+  c.vir[0] = sc[S_VIR0]; c.vir[1] = sc[S_VIR1]; c.vir[2] = sc[S_VIR2];
+}
+
+__device__ inline void store_sc(const Sc &c, double *sc) {
+  sc[S_S] = c.s; sc[S_PS] = c.Ps; sc[S_Q] = c.Q;
+  sc[S_PV0] = c.Pv[0]; sc[S_PV1] = c.Pv[1]; sc[S_PV2] = c.Pv[2]; sc[S_W] = c.W;
+  sc[S_L0] = c.L[0]; sc[S_L1] = c.L[1]; sc[S_L2] = c.L[2];
+  sc[S_T] = c.T; sc[S_P] = c.P; sc[S_GKT] = c.gkT; sc[S_H0] = c.H0;
+  sc[S_LJ] = c.lj; sc[S_CLMB] = c.clmb; sc[S_WALL] = c.wall; sc[S_POT] = c.pot;
+  sc[S_VIR0] = c.vir[0]; sc[S_VIR1] = c.vir[1]; sc[S_VIR2] = c.vir[2];
+}
+
+// Molecules::GetVolume / GetBottomArea (src/molecules.h:357-376)
+__device__ inline double volume(const JobParams &J, double L0, double L1, double L2) {
+  if (J.cmode == 1) return J.bottom_area * L2;
+  if (J.cmode == 2) return L0 * L1 * J.wall_length;
+  return L0 * L1 * L2;
+}
+
+// --------------------------------------------------------------- rotations
+// body_to_space (src/rotation.h:29-48)
+__device__ inline void body_to_space(const double *a, const double *v, double *t) {
+  const double a0sq = a[0] * a[0], a1sq = a[1] * a[1], a2sq = a[2] * a[2], a3sq = a[3] * a[3];
+  const double a01 = a[0] * a[1], a02 = a[0] * a[2], a03 = a[0] * a[3];
+  const double a12 = a[1] * a[2], a13 = a[1] * a[3], a23 = a[2] * a[3];
+  t[0] = (a0sq + a1sq - a2sq - a3sq) * v[0] + 2. * (a12 - a03) * v[1] + 2. * (a13 + a02) * v[2];
+  t[1] = 2. * (a12 + a03) * v[0] + (a0sq - a1sq + a2sq - a3sq) * v[1] + 2. * (a23 - a01) * v[2];
+  t[2] = 2. * (a13 - a02) * v[0] + 2. * (a23 + a01) * v[1] + (a0sq - a1sq - a2sq + a3sq) * v[2];
+}
+
+// space_to_body (src/rotation.h:7-27)
+__device__ inline void space_to_body(const double *a, const double *v, double *t) {
+  const double a0sq = a[0] * a[0], a1sq = a[1] * a[1], a2sq = a[2] * a[2], a3sq = a[3] * a[3];
+  const double a01 = a[0] * a[1], a02 = a[0] * a[2], a03 = a[0] * a[3];
+  const double a12 = a[1] * a[2], a13 = a[1] * a[3], a23 = a[2] * a[3];
+  t[0] = (a0sq + a1sq - a2sq - a3sq) * v[0] + 2. * (a12 + a03) * v[1] + 2. * (a13 - a02) * v[2];
+  t[1] = 2. * (a12 - a03) * v[0] + (a0sq - a1sq + a2sq - a3sq) * v[1] + 2. * (a23 + a01) * v[2];
+  t[2] = 2. * (a13 + a02) * v[0] + 2. * (a23 - a01) * v[1] + (a0sq - a1sq - a2sq + a3sq) * v[2];
+}
+
+// Free rotation about body axis k (src/integrator.cxx:467-563, P1..P3 of
+// src/integrator.h:321-346).  Returns xi.
+__device__ inline double rotate(double *q, double *p, int axis, double coef_xi, double I, double h) {
+  double Pq[4], Pp[4];
+  if (axis == 0) {
+    Pq[0] = -q[1]; Pq[1] = q[0]; Pq[2] = q[3]; Pq[3] = -q[2];
+    Pp[0] = -p[1]; Pp[1] = p[0]; Pp[2] = p[3]; Pp[3] = -p[2];
+  } else if (axis == 1) {
+    Pq[0] = -q[2]; Pq[1] = -q[3]; Pq[2] = q[0]; Pq[3] = q[1];
+    Pp[0] = -p[2]; Pp[1] = -p[3]; Pp[2] = p[0]; Pp[3] = p[1];
+  } else {
+    Pq[0] = -q[3]; Pq[1] = q[2]; Pq[2] = -q[1]; Pq[3] = q[0];
+    Pp[0] = -p[3]; Pp[1] = p[2]; Pp[2] = -p[1]; Pp[3] = p[0];
+  }
+  const double xi = (p[0] * Pq[0] + p[1] * Pq[1] + p[2] * Pq[2] + p[3] * Pq[3]) * coef_xi / I;
+  double sn, c;
+  sincos(xi * h, &sn, &c);
+#pragma unroll
+  for (int d = 0; d < 4; d++) {
+    const double qn = q[d] * c + Pq[d] * sn;
+    const double pn = p[d] * c + Pp[d] * sn;
+    q[d] = qn;
+    p[d] = pn;
+  }
+  return xi;
+}
+
+// ------------------------------------------------------------- integrator
+__device__ inline int teff(const JobParams &J) { return J.tmode == 1 ? 1 : 0; }
+
+// D6 (src/integrator.cxx:634-644)
+__device__ inline void D6(const JobParams &J, Sc &c) {
+  if (teff(J)) {
+    const double tmp = 1.0 + (c.Ps / (2.0 * c.Q) * J.dthalf);
+    c.s *= tmp * tmp;
+    c.Ps /= tmp;
+  }
+}
+
+// scalar part of D5 (src/integrator.cxx:581-600); pconstant 3 is the area
+// barostat (Lx = Ly = sqrt(A), conjugate momentum in Pv[0]).
+__device__ inline void D5_scalars(const JobParams &J, Sc &c) {
+  if (teff(J)) {
+    c.Ps -= c.pot * J.dthalf;
+    if (J.pmode > 0) c.Ps -= (c.P * volume(J, c.L[0], c.L[1], c.L[2])) * J.dthalf;
+  }
+  if (J.pmode == 1)
+    c.Pv[0] += ((c.vir[0] + c.vir[1] + c.vir[2]) / (3.0 * c.L[0] * c.L[1] * c.L[2]) - c.P) * c.s * J.dthalf;
+  if (J.pmode == 2) c.Pv[2] += c.s * (c.vir[2] / c.L[2] - c.P * J.bottom_area) * J.dthalf;
+  if (J.pmode == 3) {
+    const double A = c.L[0] * c.L[1];
+    c.Pv[0] += c.s * ((c.vir[0] + c.vir[1]) / (2.0 * A) - c.P * J.wall_length) * J.dthalf;
+  }
+}
+
+// D4 (src/integrator.cxx:604-632; pconstant 1 with the Pv^2/(2W) kinetic
+// term of remdgpu.cu:2090-2095)
+__device__ inline void D4(const JobParams &J, Sc &c) {
+  if (J.pmode == 1) {
+    if (teff(J)) c.Ps -= 0.5 * c.Pv[0] * c.Pv[0] / c.W * J.dthalf;
+    const double V = volume(J, c.L[0], c.L[1], c.L[2]) + c.Pv[0] * c.s / c.W * J.dthalf;
+    const double l = cbrt(V);
+    c.L[0] = c.L[1] = c.L[2] = l;
+  }
+  if (J.pmode == 2) {
+    if (teff(J)) c.Ps -= 0.5 * c.Pv[2] * c.Pv[2] / c.W * J.dthalf;
+    c.L[2] += c.s * c.Pv[2] / c.W * J.dthalf;
+  }
+  if (J.pmode == 3) {
+    if (teff(J)) c.Ps -= 0.5 * c.Pv[0] * c.Pv[0] / c.W * J.dthalf;
+    const double A = c.L[0] * c.L[1] + c.s * c.Pv[0] / c.W * J.dthalf;
+    const double l = sqrt(A);
+    c.L[0] = c.L[1] = l;
+  }
+}
+
+// D5 on one molecule (src/integrator.cxx:564-580): v += f L s dt/2 / m,
+// p += S(q) n 2 s dt/2.
+__device__ inline void D5_mol(const JobParams &J, const Sc &c, double *v, const double *f,
+                              double *p, const double *q, const double *n) {
+  const double m = J.mass;
+  v[0] += f[0] * (c.L[0] * c.s * J.dthalf) / m;
+  v[1] += f[1] * (c.L[1] * c.s * J.dthalf) / m;
+  v[2] += f[2] * (c.L[2] * c.s * J.dthalf) / m;
+  const double cp = 2.0 * c.s * J.dthalf;
+  const double d0 = q[0] * n[0] - q[1] * n[1] - q[2] * n[2] - q[3] * n[3];
+  const double d1 = q[1] * n[0] + q[0] * n[1] - q[3] * n[2] + q[2] * n[3];
+  const double d2 = q[2] * n[0] + q[3] * n[1] + q[0] * n[2] - q[1] * n[3];
+  const double d3 = q[3] * n[0] - q[2] * n[1] + q[1] * n[2] + q[0] * n[3];
+  p[0] += d0 * cp;
+  p[1] += d1 * cp;
+  p[2] += d2 * cp;
+  p[3] += d3 * cp;
+}
+
+// ------------------------------------------------------------ properties
+// Molecules::CalcProperties (src/molecules.cxx:702-734) over the replica.
+template <int NT>
+__device__ void calc_properties(const JobParams &J, const double *mol, int N, Sc &c, double *sc,
+                                double *red) {
+  double acc[4] = {0, 0, 0, 0};
+  const double *I = J.inertia;
+  for (int i = threadIdx.x; i < N; i += NT) {
+    const double vx = mol[F_VX * N + i], vy = mol[F_VY * N + i], vz = mol[F_VZ * N + i];
+    acc[0] += (vx * vx) * J.mass;
+    acc[1] += (vy * vy) * J.mass;
+    acc[2] += (vz * vz) * J.mass;
+    const double q0 = mol[F_Q0 * N + i], q1 = mol[F_Q1 * N + i], q2 = mol[F_Q2 * N + i], q3 = mol[F_Q3 * N + i];
+    const double p0 = mol[F_P0 * N + i], p1 = mol[F_P1 * N + i], p2 = mol[F_P2 * N + i], p3 = mol[F_P3 * N + i];
+    const double coef = 0.25 / c.s;
+    double x0 = (-p0 * q1 + p1 * q0 + p2 * q3 - p3 * q2) * ((1.0 / I[0]) * coef);
+    double x1 = (-p0 * q2 - p1 * q3 + p2 * q0 + p3 * q1) * ((1.0 / I[1]) * coef);
+    double x2 = (-p0 * q3 + p1 * q2 - p2 * q1 + p3 * q0) * ((1.0 / I[2]) * coef);
+    acc[3] += I[0] * x0 * x0 * 2.0 + I[1] * x1 * x1 * 2.0 + I[2] * x2 * x2 * 2.0;
+  }
+  block_sum<NT, 4>(acc, red);
+  double tra[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const double cf = 1.0 / (c.L[d] * c.s);
+    tra[d] = acc[d] * (cf * cf * 0.5);
+  }
+  const double tr = tra[0] + tra[1] + tra[2];
+  const double rot = acc[3];
+  const double kin = tr + rot;
+  const double tsm = 0.5 * c.Ps * c.Ps / c.Q;
+  const double tsp = c.gkT * log(c.s);
+  double bst = 0.0;
+  if (J.pmode > 0) {
+    if (J.cmode == 0)
+      bst = 0.5 * (c.Pv[0] * c.Pv[0] + c.Pv[1] * c.Pv[1] + c.Pv[2] * c.Pv[2]) / c.W + c.P * c.L[0] * c.L[1] * c.L[2];
+    else if (J.cmode == 1)
+      bst = 0.5 * c.Pv[2] * c.Pv[2] / c.W + c.P * volume(J, c.L[0], c.L[1], c.L[2]);
+    else
+      bst = 0.5 * c.Pv[0] * c.Pv[0] / c.W + c.P * volume(J, c.L[0], c.L[1], c.L[2]);
+  }
+  double tot = c.pot + kin;
+  if (J.tmode == 1) tot += tsm + tsp;
+  if (J.pmode > 0) tot += bst;
+  const double V = volume(J, c.L[0], c.L[1], c.L[2]);
+  if (threadIdx.x == 0) {
+    sc[S_TRA0] = tra[0]; sc[S_TRA1] = tra[1]; sc[S_TRA2] = tra[2];
+    sc[S_TRA] = tr; sc[S_ROT] = rot; sc[S_KIN] = kin;
+    sc[S_TSM] = tsm; sc[S_TSP] = tsp; sc[S_TST] = tsm + tsp; sc[S_BST] = bst;
+    sc[S_TOT] = tot; sc[S_HAM] = tot - c.H0;
+    sc[S_TK] = kin * J.unit_temp / (3. * N);
+    for (int d = 0; d < 3; d++) sc[S_PRS0 + d] = (tra[d] * 2.0 + c.vir[d]) / V * J.unit_press;
+    sc[S_PRS3] = ((tra[0] * 2.0 + c.vir[0]) + (tra[1] * 2.0 + c.vir[1]) + (tra[2] * 2.0 + c.vir[2])) /
+                 (3. * V) * J.unit_press;
+    sc[S_VOL] = V;
+  }
+  __syncthreads();
+}
+
+// Returns tot (same value on every thread) without writing sc.
+template <int NT>
+__device__ double total_energy(const JobParams &J, const double *mol, int N, const Sc &c, double *red) {
+  double acc[4] = {0, 0, 0, 0};
+  const double *I = J.inertia;
+  for (int i = threadIdx.x; i < N; i += NT) {
+    const double vx = mol[F_VX * N + i], vy = mol[F_VY * N + i], vz = mol[F_VZ * N + i];
+    acc[0] += (vx * vx) * J.mass;
+    acc[1] += (vy * vy) * J.mass;
+    acc[2] += (vz * vz) * J.mass;
+    const double q0 = mol[F_Q0 * N + i], q1 = mol[F_Q1 * N + i], q2 = mol[F_Q2 * N + i], q3 = mol[F_Q3 * N + i];
+    const double p0 = mol[F_P0 * N + i], p1 = mol[F_P1 * N + i], p2 = mol[F_P2 * N + i], p3 = mol[F_P3 * N + i];
+    const double coef = 0.25 / c.s;
+    double x0 = (-p0 * q1 + p1 * q0 + p2 * q3 - p3 * q2) * ((1.0 / I[0]) * coef);
+    double x1 = (-p0 * q2 - p1 * q3 + p2 * q0 + p3 * q1) * ((1.0 / I[1]) * coef);
+    double x2 = (-p0 * q3 + p1 * q2 - p2 * q1 + p3 * q0) * ((1.0 / I[2]) * coef);
+    acc[3] += I[0] * x0 * x0 * 2.0 + I[1] * x1 * x1 * 2.0 + I[2] * x2 * x2 * 2.0;
+  }
+  block_sum<NT, 4>(acc, red);
+  double tra = 0.0;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const double cf = 1.0 / (c.L[d] * c.s);
+    tra += acc[d] * (cf * cf * 0.5);
+  }
+  const double kin = tra + acc[3];
+  double tot = c.pot + kin;
+  if (J.tmode == 1) tot += 0.5 * c.Ps * c.Ps / c.Q + c.gkT * log(c.s);
+  if (J.pmode > 0) {
+    if (J.cmode == 0)
+      tot += 0.5 * (c.Pv[0] * c.Pv[0] + c.Pv[1] * c.Pv[1] + c.Pv[2] * c.Pv[2]) / c.W + c.P * c.L[0] * c.L[1] * c.L[2];
+    else if (J.cmode == 1)
+      tot += 0.5 * c.Pv[2] * c.Pv[2] / c.W + c.P * volume(J, c.L[0], c.L[1], c.L[2]);
+    else
+      tot += 0.5 * c.Pv[0] * c.Pv[0] / c.W + c.P * volume(J, c.L[0], c.L[1], c.L[2]);
+  }
+  return tot;
+}
+
+// KillMomentum + KillAngularMomentumInTube (src/molecules.cxx:168-210)
+template <int NT>
+__device__ void kill_momentum(const JobParams &J, double *mol, int N, const Sc &c, double *red,
+                              bool tube_twice) {
+  double acc[4] = {0, 0, 0, 0};
+  for (int i = threadIdx.x; i < N; i += NT) {
+    acc[0] += mol[F_VX * N + i] * J.mass;
+    acc[1] += mol[F_VY * N + i] * J.mass;
+    acc[2] += mol[F_VZ * N + i] * J.mass;
+    acc[3] += J.mass;
+  }
+  block_sum<NT, 4>(acc, red);
+  const double sum[3] = {acc[0] / acc[3], acc[1] / acc[3], acc[2] / acc[3]};
+  for (int i = threadIdx.x; i < N; i += NT) {
+    if (J.cmode == 1) {
+      mol[F_VZ * N + i] -= sum[2];
+    } else if (J.cmode == 2) {
+      mol[F_VX * N + i] -= sum[0];
+      mol[F_VY * N + i] -= sum[1];
+    } else {
+      mol[F_VX * N + i] -= sum[0];
+      mol[F_VY * N + i] -= sum[1];
+      mol[F_VZ * N + i] -= sum[2];
+    }
+  }
+  __syncthreads();
+  const int reps = (J.cmode == 1) ? (tube_twice ? 2 : 1) : 0;
+  for (int rpt = 0; rpt < reps; rpt++) {
+    double a2[2] = {0, 0};
+    for (int i = threadIdx.x; i < N; i += NT) {
+      const double rx = (mol[F_RX * N + i] - 0.5) * c.L[0];
+      const double ry = (mol[F_RY * N + i] - 0.5) * c.L[1];
+      const double vx = mol[F_VX * N + i] / (c.L[0] * c.s);
+      const double vy = mol[F_VY * N + i] / (c.L[1] * c.s);
+      a2[0] += J.mass * (rx * vy - ry * vx);
+      a2[1] += J.mass;
+    }
+    block_sum<NT, 2>(a2, red);
+    const double mom = a2[0] / a2[1];
+    for (int i = threadIdx.x; i < N; i += NT) {
+      const double rx = (mol[F_RX * N + i] - 0.5) * c.L[0];
+      const double ry = (mol[F_RY * N + i] - 0.5) * c.L[1];
+      const double r02 = rx * rx + ry * ry;
+      if (r02 > 1.0) {
+        mol[F_VX * N + i] += mom * ry / r02 * c.L[0] * c.s;
+        mol[F_VY * N + i] -= mom * rx / r02 * c.L[1] * c.s;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// VelocityScaling + AngVelocityScaling (src/molecules.cxx:261-311)
+template <int NT>
+__device__ void velocity_scaling(const JobParams &J, double *mol, int N, Sc &c, double *red) {
+  double acc[3] = {0, 0, 0};
+  for (int i = threadIdx.x; i < N; i += NT) {
+    acc[0] += mol[F_VX * N + i] * mol[F_VX * N + i] * J.mass;
+    acc[1] += mol[F_VY * N + i] * mol[F_VY * N + i] * J.mass;
+    acc[2] += mol[F_VZ * N + i] * mol[F_VZ * N + i] * J.mass;
+  }
+  block_sum<NT, 3>(acc, red);
+  double tra = 0.0;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const double cf = 1.0 / (c.L[d] * c.s);
+    tra += acc[d] * (cf * cf * 0.5);
+  }
+  if (tra != 0.0) {
+    const double coef = sqrt(1.5 * N * c.T / tra);
+    for (int i = threadIdx.x; i < N; i += NT) {
+      mol[F_VX * N + i] *= coef;
+      mol[F_VY * N + i] *= coef;
+      mol[F_VZ * N + i] *= coef;
+    }
+    __syncthreads();
+    kill_momentum<NT>(J, mol, N, c, red, true);
+    c.s = 1.0;
+    c.Ps = 0.0;
+    c.Pv[0] = c.Pv[1] = c.Pv[2] = 0.0;
+  }
+  // AngVelocityScaling
+  double rr[1] = {0.0};
+  const double *I = J.inertia;
+  for (int i = threadIdx.x; i < N; i += NT) {
+    const double q0 = mol[F_Q0 * N + i], q1 = mol[F_Q1 * N + i], q2 = mol[F_Q2 * N + i], q3 = mol[F_Q3 * N + i];
+    const double p0 = mol[F_P0 * N + i], p1 = mol[F_P1 * N + i], p2 = mol[F_P2 * N + i], p3 = mol[F_P3 * N + i];
+    const double coef = 0.25 / c.s;
+    double x0 = (-p0 * q1 + p1 * q0 + p2 * q3 - p3 * q2) * ((1.0 / I[0]) * coef);
+    double x1 = (-p0 * q2 - p1 * q3 + p2 * q0 + p3 * q1) * ((1.0 / I[1]) * coef);
+    double x2 = (-p0 * q3 + p1 * q2 - p2 * q1 + p3 * q0) * ((1.0 / I[2]) * coef);
+    rr[0] += I[0] * x0 * x0 * 2.0 + I[1] * x1 * x1 * 2.0 + I[2] * x2 * x2 * 2.0;
+  }
+  block_sum<NT, 1>(rr, red);
+  if (rr[0] != 0.0) {
+    const double coef = sqrt(1.5 * N * c.T / rr[0]);
+    for (int i = threadIdx.x; i < N; i += NT)
+      for (int d = 0; d < 4; d++) mol[(F_P0 + d) * N + i] *= coef;
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------ neighbour build
+// Wave-ballot LSD radix sort of 64-bit keys with 16-bit payloads, 4-bit
+// digits, stable: within a 64-lane group the rank is the popcount of the
+// lanes below with the same digit; groups are scanned in element order.
+template <int NT>
+__device__ void radix_sort(uint64_t *keyA, uint16_t *idxA, uint64_t *keyB, uint16_t *idxB, int *cnt,
+                           int n) {
+  constexpr int NW = NT / WAVE;
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int w = threadIdx.x / WAVE;
+  const int ntiles = (n + NT - 1) / NT;
+  const int ng = ntiles * NW;
+  const uint64_t below = (lane == 0) ? 0ull : ((1ull << lane) - 1ull);
+  for (int pass = 0; pass < 16; ++pass) {
+    const int shift = 4 * pass;
+    for (int t = 0; t < ntiles; ++t) {
+      const int e = t * NT + threadIdx.x;
+      const bool valid = e < n;
+      const int dg = valid ? (int)((keyA[e] >> shift) & 15ull) : 16;
+      const int g = t * NW + w;
+      for (int v = 0; v < 16; ++v) {
+        const uint64_t b = __ballot(dg == v);
+        if (lane == v) cnt[v * ng + g] = __popcll(b);
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < 16) {
+      const int v = threadIdx.x;
+      int s = 0;
+      for (int g = 0; g < ng; ++g) {
+        const int c = cnt[v * ng + g];
+        cnt[v * ng + g] = s;
+        s += c;
+      }
+      cnt[16 * ng + v] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int s = 0;
+      for (int v = 0; v < 16; ++v) {
+        const int c = cnt[16 * ng + v];
+        cnt[16 * ng + v] = s;
+        s += c;
+      }
+    }
+    __syncthreads();
+    for (int t = 0; t < ntiles; ++t) {
+      const int e = t * NT + threadIdx.x;
+      const bool valid = e < n;
+      const uint64_t key = valid ? keyA[e] : 0ull;
+      const uint16_t id = valid ? idxA[e] : (uint16_t)0;
+      const int dg = (int)((key >> shift) & 15ull);
+      uint64_t m = __ballot(valid);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int bit = (dg >> k) & 1;
+        const uint64_t b = __ballot(valid && bit);
+        m &= bit ? b : ~b;
+      }
+      const int rank = __popcll(m & below);
+      const int g = t * NW + w;
+      if (valid) {
+        const int pos = cnt[16 * ng + dg] + cnt[dg * ng + g] + rank;
+        keyB[pos] = key;
+        idxB[pos] = id;
+      }
+    }
+    __syncthreads();
+    uint64_t *tk = keyA; keyA = keyB; keyB = tk;
+    uint16_t *ti = idxA; idxA = idxB; idxB = ti;
+  }
+}
+
+// Window bounds for nlane = 1 (src/molecules.cxx:556-583) on sorted keys z:
+// js = first index scanning down from i (cyclic) with z < z_i - w,
+// je = first index scanning up (cyclic) with z > z_i + w.
+__device__ inline int count_less(const double *z, int n, double x) {
+  int lo = 0, hi = n;  // first index with z >= x
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (z[mid] < x) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+__device__ inline int count_leq(const double *z, int n, double x) {
+  int lo = 0, hi = n;  // first index with z > x
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (z[mid] <= x) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+__device__ inline void window_of(const double *z, int n, int i, double w, int &js, int &je) {
+  double lb = z[i] - w;
+  int k = count_less(z, n, lb);
+  if (k >= 1) js = k - 1;
+  else {
+    lb += 1.0;
+    js = count_less(z, n, lb) - 1;
+  }
+  double ub = z[i] + w;
+  int m = count_leq(z, n, ub);
+  if (m < n) je = m;
+  else {
+    ub -= 1.0;
+    je = count_leq(z, n, ub);
+  }
+}
+
+// Sort the replica's molecules by the sort-axis coordinate, permute every
+// state array, rebuild js/je (CalcForcePot's doSort branch,
+// src/molecules.cxx:543-585).
+template <int NT>
+__device__ void sort_and_window(const JobParams &J, double *mol, int *mid, int N, double Laxis,
+                                double margin, const Lds &S, int *win_dbg) {
+  uint64_t *keyA = (uint64_t *)S.region0;
+  uint64_t *keyB = keyA + N;
+  uint16_t *idxA = (uint16_t *)(keyB + N);
+  uint16_t *idxB = idxA + N;
+  const double *ax = mol + (F_RX + J.sort_axis) * N;
+  for (int i = threadIdx.x; i < N; i += NT) {
+    const double z = ax[i] + 0.0;  // -0.0 -> +0.0
+    keyA[i] = (uint64_t)__double_as_longlong(z);
+    idxA[i] = (uint16_t)i;
+  }
+  __syncthreads();
+  radix_sort<NT>(keyA, idxA, keyB, idxB, S.cnt, N);
+  // permute the molecule block through LDS (keyB area as staging)
+  double *tmp = (double *)keyB;
+  for (int f = 0; f < NFIELD; ++f) {
+    double *col = mol + f * N;
+    for (int i = threadIdx.x; i < N; i += NT) tmp[i] = col[i];
+    __syncthreads();
+    for (int k = threadIdx.x; k < N; k += NT) col[k] = tmp[idxA[k]];
+    __syncthreads();
+  }
+  {
+    int *tmpi = (int *)keyB;
+    for (int i = threadIdx.x; i < N; i += NT) tmpi[i] = mid[i];
+    __syncthreads();
+    for (int k = threadIdx.x; k < N; k += NT) mid[k] = tmpi[idxA[k]];
+    __syncthreads();
+  }
+  // sorted coordinate into LDS, then the windows
+  double *zs = (double *)keyA;
+  for (int i = threadIdx.x; i < N; i += NT) zs[i] = ax[i];
+  __syncthreads();
+  const double w = (J.rcut + margin) / Laxis;
+  for (int i = threadIdx.x; i < N; i += NT) {
+    int js, je;
+    window_of(zs, N, i, w, js, je);
+    if (win_dbg) {
+      win_dbg[2 * i] = js;
+      win_dbg[2 * i + 1] = je;
+    }
+    if (js >= je) je += N;
+    // a scanned range that misses i means the window spans the whole box:
+    // every other molecule is a candidate
+    if (!((js <= i && i <= je) || (js <= i + N && i + N <= je))) {
+      js = i + 1;
+      je = i + N - 1;
+    }
+    S.js[i] = (uint16_t)js;
+    S.je[i] = (uint16_t)je;
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------- forces
+template <int NS, typename PT>
+struct PairK {
+  PT rc, rl, rc2, rl2, coef, coef30;
+  PT sig2, eps4, eps48;
+  PT qq[NS][NS];
+  PT L[3], Li[3];
+  int per[3];
+};
+
+template <int NS>
+struct Acc {
+  double site[NS][3];
+  double corr[3];
+  double vir[3];
+  double lj, clmb;
+};
+
+__device__ inline float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ inline double rsq(double x) { return 1.0 / sqrt(x); }
+__device__ inline float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline double rcp(double x) { return 1.0 / x; }
+__device__ inline float rnd(float x) { return rintf(x); }
+__device__ inline double rnd(double x) { return rint(x); }
+
+// One ordered molecule pair (i, j): switched LJ (site 0 - site 0) + Coulomb
+// over charged site pairs, CPU tuned-kernel semantics
+// (src/force_calculator.cxx:874-989).  Pair math in PT, accumulation in FP64.
+template <int NS, int QB, typename PT>
+__device__ __forceinline__ void pair_acc(const PT *gi, const PT (&ai)[NS][3], const PT *rj,
+                                         const PairK<NS, PT> &k, Acc<NS> &acc) {
+  PT dg[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    dg[d] = gi[d] - rj[d];
+    if (k.per[d]) dg[d] -= k.L[d] * rnd(dg[d] * k.Li[d]);
+  }
+  const PT r2 = dg[0] * dg[0] + dg[1] * dg[1] + dg[2] * dg[2];
+  if (!(r2 < k.rc2)) return;
+  PT sw = PT(1), dswr = PT(0);
+  if (r2 > k.rl2) {
+    const PT ri = rsq(r2);
+    const PT r = r2 * ri;
+    const PT c_ = r - k.rc;
+    const PT l_ = r - k.rl;
+    sw = k.coef * c_ * c_ * c_ * (PT(10) * l_ * l_ - PT(5) * l_ * c_ + c_ * c_);
+    dswr = k.coef30 * c_ * c_ * l_ * l_ * ri;
+  }
+  PT di[NS][3];
+#pragma unroll
+  for (int a = 0; a < NS; a++)
+#pragma unroll
+    for (int d = 0; d < 3; d++) di[a][d] = dg[d] + ai[a][d];
+  PT fs[NS][3];
+#pragma unroll
+  for (int a = 0; a < NS; a++) fs[a][0] = fs[a][1] = fs[a][2] = PT(0);
+  PT elj, ecl = PT(0);
+  {  // LJ between the two site-0 (oxygen) sites
+    const PT x = di[0][0] - rj[3], y = di[0][1] - rj[4], z = di[0][2] - rj[5];
+    const PT rr = x * x + y * y + z * z;
+    const PT ri2 = rcp(rr);
+    const PT s2 = k.sig2 * ri2;
+    const PT s6 = s2 * s2 * s2;
+    elj = k.eps4 * s6 * (s6 - PT(1));
+    const PT f = k.eps48 * s6 * (s6 - PT(0.5)) * ri2;
+    fs[0][0] += f * x;
+    fs[0][1] += f * y;
+    fs[0][2] += f * z;
+  }
+#pragma unroll
+  for (int a = QB; a < NS; a++) {
+#pragma unroll
+    for (int b = QB; b < NS; b++) {
+      const PT x = di[a][0] - rj[3 + 3 * b], y = di[a][1] - rj[4 + 3 * b], z = di[a][2] - rj[5 + 3 * b];
+      const PT rr = x * x + y * y + z * z;
+      const PT ri = rsq(rr);
+      const PT e = k.qq[a][b] * ri;
+      const PT f = e * ri * ri;
+      ecl += e;
+      fs[a][0] += f * x;
+      fs[a][1] += f * y;
+      fs[a][2] += f * z;
+    }
+  }
+  const PT cr = -dswr * (elj + ecl);
+  PT F0 = cr * dg[0], F1 = cr * dg[1], F2 = cr * dg[2];
+  acc.corr[0] += (double)F0;
+  acc.corr[1] += (double)F1;
+  acc.corr[2] += (double)F2;
+#pragma unroll
+  for (int a = 0; a < NS; a++) {
+    const PT sx = sw * fs[a][0], sy = sw * fs[a][1], sz = sw * fs[a][2];
+    acc.site[a][0] += (double)sx;
+    acc.site[a][1] += (double)sy;
+    acc.site[a][2] += (double)sz;
+    F0 += sx;
+    F1 += sy;
+    F2 += sz;
+  }
+  acc.vir[0] += (double)(F0 * dg[0]);
+  acc.vir[1] += (double)(F1 * dg[1]);
+  acc.vir[2] += (double)(F2 * dg[2]);
+  acc.lj += (double)(sw * elj);
+  acc.clmb += (double)(sw * ecl);
+}
+
+// M2A: LDS records (COM centred on the box, site offsets R(q) b).
+template <int NT, int NS, typename PT>
+__device__ void m2a(const JobParams &J, const double *mol, int N, double L0, double L1, double L2, PT *rec) {
+  constexpr int ST = Rec<NS>::STRIDE;
+  for (int i = threadIdx.x; i < N; i += NT) {
+    PT *o = rec + i * ST;
+    o[0] = (PT)((mol[F_RX * N + i] - 0.5) * L0);
+    o[1] = (PT)((mol[F_RY * N + i] - 0.5) * L1);
+    o[2] = (PT)((mol[F_RZ * N + i] - 0.5) * L2);
+    const double q[4] = {mol[F_Q0 * N + i], mol[F_Q1 * N + i], mol[F_Q2 * N + i], mol[F_Q3 * N + i]};
+#pragma unroll
+    for (int a = 0; a < NS; a++) {
+      double t[3];
+      body_to_space(q, J.site_r[a], t);
+      o[3 + 3 * a] = (PT)t[0];
+      o[4 + 3 * a] = (PT)t[1];
+      o[5 + 3 * a] = (PT)t[2];
+    }
+  }
+}
+
+// Pair pass + wall + A2M for every molecule of the replica.  Writes f, n
+// into the molecule block and the energy/virial totals into c.
+template <int NT, int NS, typename PT>
+__device__ void force_pass(const JobParams &J, double *mol, int N, Sc &c, const Lds &S,
+                           double *site_dbg) {
+  constexpr int QB = (NS == 3) ? 0 : 1;
+  constexpr int ST = Rec<NS>::STRIDE;
+  PT *rec = (PT *)S.region0;
+  m2a<NT, NS, PT>(J, mol, N, c.L[0], c.L[1], c.L[2], rec);
+  __syncthreads();
+
+  PairK<NS, PT> k;
+  k.rc = (PT)J.rcut;
+  k.rl = (PT)J.rswitch;
+  k.rc2 = (PT)(J.rcut * J.rcut);
+  k.rl2 = (PT)(J.rswitch * J.rswitch);
+  {
+    const double rlc = J.rswitch - J.rcut;
+    const double coef = 1.0 / (rlc * rlc * rlc * rlc * rlc);
+    k.coef = (PT)coef;
+    k.coef30 = (PT)(30.0 * coef);
+  }
+  k.sig2 = (PT)J.lj_sig2;
+  k.eps4 = (PT)J.lj_eps4;
+  k.eps48 = (PT)(12.0 * J.lj_eps4);
+#pragma unroll
+  for (int a = 0; a < NS; a++)
+#pragma unroll
+    for (int b = 0; b < NS; b++) k.qq[a][b] = (PT)(J.q[a] * J.q[b]);
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    k.L[d] = (PT)c.L[d];
+    k.Li[d] = (PT)(1.0 / c.L[d]);
+    k.per[d] = J.per[d];
+  }
+
+  const int C = J.chunks;
+  const int lc = __ffs(C) - 1;  // log2(C)
+  double e_lj = 0.0, e_cl = 0.0, e_wall = 0.0, vir[3] = {0.0, 0.0, 0.0};
+  const double dr = (J.cmode == 1) ? J.wall_length / (double)J.nfwall
+                                   : 0.5 * J.wall_length / (double)J.nfwall;
+  const double dri = 1.0 / dr;
+  const int total = N << lc;
+  for (int base = 0; base < total; base += NT) {
+    const int t = base + threadIdx.x;
+    const bool active = t < total;
+    const int i = active ? (t >> lc) : 0;
+    const int cc = t & (C - 1);
+    Acc<NS> acc;
+#pragma unroll
+    for (int a = 0; a < NS; a++) acc.site[a][0] = acc.site[a][1] = acc.site[a][2] = 0.0;
+    acc.corr[0] = acc.corr[1] = acc.corr[2] = 0.0;
+    acc.vir[0] = acc.vir[1] = acc.vir[2] = 0.0;
+    acc.lj = acc.clmb = 0.0;
+    if (active) {
+      const PT *ri = rec + i * ST;
+      PT gi[3] = {ri[0], ri[1], ri[2]};
+      PT ai[NS][3];
+#pragma unroll
+      for (int a = 0; a < NS; a++) {
+        ai[a][0] = ri[3 + 3 * a];
+        ai[a][1] = ri[4 + 3 * a];
+        ai[a][2] = ri[5 + 3 * a];
+      }
+      const int js = S.js[i];
+      const int je = S.je[i];
+      for (int jj = js + cc; jj <= je; jj += C) {
+        const int j = (jj >= N) ? jj - N : jj;
+        if (j == i) continue;
+        PT rj[3 + 3 * NS];
+        const PT *src = rec + j * ST;
+#pragma unroll
+        for (int u = 0; u < 3 + 3 * NS; u++) rj[u] = src[u];
+        pair_acc<NS, QB, PT>(gi, ai, rj, k, acc);
+      }
+    }
+    // fixed xor butterfly over the C lanes of molecule i
+    for (int m = 1; m < C; m <<= 1) {
+#pragma unroll
+      for (int a = 0; a < NS; a++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) acc.site[a][d] += __shfl_xor(acc.site[a][d], m, WAVE);
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+        acc.corr[d] += __shfl_xor(acc.corr[d], m, WAVE);
+        acc.vir[d] += __shfl_xor(acc.vir[d], m, WAVE);
+      }
+      acc.lj += __shfl_xor(acc.lj, m, WAVE);
+      acc.clmb += __shfl_xor(acc.clmb, m, WAVE);
+    }
+    if (active && cc == 0) {
+      // wall on the site-0 (oxygen) position, FP64
+      const double q[4] = {mol[F_Q0 * N + i], mol[F_Q1 * N + i], mol[F_Q2 * N + i], mol[F_Q3 * N + i]};
+      double F[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+        double s = acc.corr[d];
+#pragma unroll
+        for (int a = 0; a < NS; a++) s += acc.site[a][d];
+        F[d] = s;
+      }
+      if (J.cmode == 1 || J.cmode == 2) {
+        double b0[3];
+        body_to_space(q, J.site_r[0], b0);
+        if (J.cmode == 1) {
+          const double x = (mol[F_RX * N + i] - 0.5) * c.L[0] + b0[0];
+          const double y = (mol[F_RY * N + i] - 0.5) * c.L[1] + b0[1];
+          const double r01 = sqrt(x * x + y * y);
+          int idx = (int)(r01 / dr);
+          if (idx > J.nfwall - 2) idx = J.nfwall - 2;
+          const double f = (J.fwall[2 * (idx + 1)] - J.fwall[2 * idx]) * dri * (r01 - dr * idx) + J.fwall[2 * idx];
+          const double e = (J.fwall[2 * (idx + 1) + 1] - J.fwall[2 * idx + 1]) * dri * (r01 - dr * idx) + J.fwall[2 * idx + 1];
+          acc.site[0][0] += f * x;
+          acc.site[0][1] += f * y;
+          F[0] += f * x;
+          F[1] += f * y;
+          e_wall += e;
+        } else {
+          const double z = (mol[F_RZ * N + i] - 0.5) * c.L[2] + b0[2];
+          const double az = fabs(z);
+          int idx = (int)(az / dr);
+          if (idx > J.nfwall - 2) idx = J.nfwall - 2;
+          const double f = (J.fwall[2 * (idx + 1)] - J.fwall[2 * idx]) * dri * (az - dr * idx) + J.fwall[2 * idx];
+          const double e = (J.fwall[2 * (idx + 1) + 1] - J.fwall[2 * idx + 1]) * dri * (az - dr * idx) + J.fwall[2 * idx + 1];
+          acc.site[0][2] += f * z;
+          F[2] += f * z;
+          e_wall += e;
+        }
+      }
+      // A2M torque (src/molecules.cxx:409-418)
+      double n4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int a = 0; a < NS; a++) {
+        double fb[3];
+        space_to_body(q, acc.site[a], fb);
+        const double *br = J.site_r[a];
+        n4[0] += br[0] * fb[0] + br[1] * fb[1] + br[2] * fb[2];
+        n4[1] += br[1] * fb[2] - br[2] * fb[1];
+        n4[2] += br[2] * fb[0] - br[0] * fb[2];
+        n4[3] += br[0] * fb[1] - br[1] * fb[0];
+      }
+      mol[F_FX * N + i] = F[0];
+      mol[F_FY * N + i] = F[1];
+      mol[F_FZ * N + i] = F[2];
+      mol[F_N0 * N + i] = n4[0];
+      mol[F_N1 * N + i] = n4[1];
+      mol[F_N2 * N + i] = n4[2];
+      mol[F_N3 * N + i] = n4[3];
+      if (site_dbg) {
+#pragma unroll
+        for (int a = 0; a < NS; a++)
+#pragma unroll
+          for (int d = 0; d < 3; d++) site_dbg[(i * NS + a) * 3 + d] = acc.site[a][d];
+      }
+      e_lj += acc.lj;
+      e_cl += acc.clmb;
+      vir[0] += acc.vir[0];
+      vir[1] += acc.vir[1];
+      vir[2] += acc.vir[2];
+    }
+  }
+  double tot[6] = {e_lj, e_cl, e_wall, vir[0], vir[1], vir[2]};
+  block_sum<NT, 6>(tot, S.red);
+  c.lj = 0.5 * tot[0];
+  c.clmb = 0.5 * tot[1];
+  c.wall = tot[2];
+  c.vir[0] = 0.5 * tot[3];
+  c.vir[1] = 0.5 * tot[4];
+  c.vir[2] = 0.5 * tot[5];
+  c.pot = c.lj + c.clmb + c.wall;
+}
+
+// ----------------------------------------------------------- one step
+template <int NT, int NS, typename PT>
+__device__ void md_step(const JobParams &J, double *mol, int *mid, int N, Sc &c, const Lds &S,
+                        bool do_sort, const ReplicaDesc &R) {
+  // D6, D5 scalars, D4 (src/integrator.cxx:651-658)
+  D6(J, c);
+
+  Sc c_old = c;
+  D5_scalars(J, c);
+  D4(J, c);
+  // per molecule: D5 (old box), D3 D2 D1 D2 D3 (new box)
+  const bool T1 = teff(J) == 1;
+  const bool needPv = (J.pmode > 0) || T1;
+  double sums[4] = {0, 0, 0, 0};  // sum_Pv[3], sum_Ps
+  {
+    const double coef_xi = 0.25 / c.s;
+    const double si = 1.0 / c.s;
+    double coef_r[3];
+  #pragma unroll
+  for (int d = 0; d < 3; d++) {
+      const double Li = 1.0 / c.L[d];
+      coef_r[d] = Li * Li * si * J.dt;
+    }
+    const double *I = J.inertia;
+    for (int i = threadIdx.x; i < N; i += NT) {
+      double r[3] = {mol[F_RX * N + i], mol[F_RY * N + i], mol[F_RZ * N + i]};
+      double v[3] = {mol[F_VX * N + i], mol[F_VY * N + i], mol[F_VZ * N + i]};
+      const double f[3] = {mol[F_FX * N + i], mol[F_FY * N + i], mol[F_FZ * N + i]};
+      double q[4] = {mol[F_Q0 * N + i], mol[F_Q1 * N + i], mol[F_Q2 * N + i], mol[F_Q3 * N + i]};
+      double p[4] = {mol[F_P0 * N + i], mol[F_P1 * N + i], mol[F_P2 * N + i], mol[F_P3 * N + i]};
+      const double n[4] = {mol[F_N0 * N + i], mol[F_N1 * N + i], mol[F_N2 * N + i], mol[F_N3 * N + i]};
+      D5_mol(J, c_old, v, f, p, q, n);
+      double xi;
+      xi = rotate(q, p, 2, coef_xi, I[2], J.dthalf);
+      if (T1) sums[3] += 0.5 * xi * xi * I[2];
+      xi = rotate(q, p, 1, coef_xi, I[1], J.dthalf);
+      if (T1) sums[3] += 0.5 * xi * xi * I[1];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+        r[d] += v[d] * coef_r[d];
+        if (r[d] < 0.0) r[d] += 1.0;
+        if (r[d] >= 1.0) r[d] -= 1.0;
+      }
+      xi = rotate(q, p, 0, 0.25 * si, I[0], J.dt);
+      if (needPv) {
+        sums[0] += 2.0 * J.mass * v[0] * v[0];
+        sums[1] += 2.0 * J.mass * v[1] * v[1];
+        sums[2] += 2.0 * J.mass * v[2] * v[2];
+      }
+      if (T1) sums[3] += xi * xi * I[0];
+      xi = rotate(q, p, 1, coef_xi, I[1], J.dthalf);
+      if (T1) sums[3] += 0.5 * xi * xi * I[1];
+      xi = rotate(q, p, 2, coef_xi, I[2], J.dthalf);
+      if (T1) sums[3] += 0.5 * xi * xi * I[2];
+      mol[F_RX * N + i] = r[0]; mol[F_RY * N + i] = r[1]; mol[F_RZ * N + i] = r[2];
+      mol[F_VX * N + i] = v[0]; mol[F_VY * N + i] = v[1]; mol[F_VZ * N + i] = v[2];
+      mol[F_Q0 * N + i] = q[0]; mol[F_Q1 * N + i] = q[1]; mol[F_Q2 * N + i] = q[2]; mol[F_Q3 * N + i] = q[3];
+      mol[F_P0 * N + i] = p[0]; mol[F_P1 * N + i] = p[1]; mol[F_P2 * N + i] = p[2]; mol[F_P3 * N + i] = p[3];
+    }
+  }
+
+  block_sum<NT, 4>(sums, S.red);  // includes __syncthreads: molecule writes visible
+  // PsPv (src/integrator.cxx:145-171)
+  if (needPv) {
+    for (int d = 0; d < 3; d++) sums[d] /= c.L[d] * c.L[d] * c.s * c.s;
+  }
+  if (J.pmode == 1) c.Pv[0] += c.s * (sums[0] + sums[1] + sums[2]) / (volume(J, c.L[0], c.L[1], c.L[2]) * 3.0) * J.dthalf;
+  if (J.pmode == 2) c.Pv[2] += c.s * sums[2] / c.L[2] * J.dthalf;
+  if (J.pmode == 3) {
+    const double A = c.L[0] * c.L[1];
+    c.Pv[0] += c.s * (sums[0] + sums[1]) / (2.0 * A) * J.dthalf;
+  }
+  if (T1) c.Ps += (0.25 * (sums[0] + sums[1] + sums[2]) + 2.0 * sums[3] - (1.0 + log(c.s)) * c.gkT + c.H0) * J.dt;
+  D4(J, c);
+  // force
+  if (do_sort) sort_and_window<NT>(J, mol, mid, N, J.sort_axis == 0 ? c.L[0] : c.L[2], J.margin, S, R.win);
+  force_pass<NT, NS, PT>(J, mol, N, c, S, R.site_f);  // ends with block_sum (synced)
+  // D5, D6
+  for (int i = threadIdx.x; i < N; i += NT) {
+    double v[3] = {mol[F_VX * N + i], mol[F_VY * N + i], mol[F_VZ * N + i]};
+    const double f[3] = {mol[F_FX * N + i], mol[F_FY * N + i], mol[F_FZ * N + i]};
+    const double q[4] = {mol[F_Q0 * N + i], mol[F_Q1 * N + i], mol[F_Q2 * N + i], mol[F_Q3 * N + i]};
+    double p[4] = {mol[F_P0 * N + i], mol[F_P1 * N + i], mol[F_P2 * N + i], mol[F_P3 * N + i]};
+    const double n[4] = {mol[F_N0 * N + i], mol[F_N1 * N + i], mol[F_N2 * N + i], mol[F_N3 * N + i]};
+    D5_mol(J, c, v, f, p, q, n);
+    mol[F_VX * N + i] = v[0]; mol[F_VY * N + i] = v[1]; mol[F_VZ * N + i] = v[2];
+    mol[F_P0 * N + i] = p[0]; mol[F_P1 * N + i] = p[1]; mol[F_P2 * N + i] = p[2]; mol[F_P3 * N + i] = p[3];
+  }
+  D5_scalars(J, c);
+  D6(J, c);
+  __syncthreads();
+}
+
+// --------------------------------------------------------- replica driver
+template <int NT, int NS, typename PT>
+__device__ void run_replica(const LaunchArgs &a, const ReplicaDesc &R, const JobParams &J, char *smem) {
+  const int N = J.nmol;
+  double *mol = R.mol;
+  int *mid = R.mid;
+  Lds S = lds_carve<NT>(smem, N, NS, J.precision);
+  Sc c;
+  load_sc(c, R.sc);
+
+  if (a.mode == MODE_INIT) {
+    // InitializeProperty (src/molecules.h:260-270): sort with margin 0,
+    // force, gkT, CalcProperties, H0 = tot.
+    sort_and_window<NT>(J, mol, mid, N, J.sort_axis == 0 ? c.L[0] : c.L[2], 0.0, S, R.win);
+    force_pass<NT, NS, PT>(J, mol, N, c, S, R.site_f);
+    c.gkT = 6.0 * (double)N * c.T;
+    const double tot = total_energy<NT>(J, mol, N, c, S.red);
+    c.H0 = tot;
+    calc_properties<NT>(J, mol, N, c, R.sc, S.red);
+    if (threadIdx.x == 0) store_sc(c, R.sc);
+    return;
+  }
+  if (a.mode == MODE_FORCE) {
+    if (a.do_sort) sort_and_window<NT>(J, mol, mid, N, J.sort_axis == 0 ? c.L[0] : c.L[2], a.margin, S, R.win);
+    else {
+      // reuse the stored window
+      for (int i = threadIdx.x; i < N; i += NT) {
+        int js = R.win ? R.win[2 * i] : 0;
+        int je = R.win ? R.win[2 * i + 1] : N - 1;
+        if (js >= je) je += N;
+        if (!((js <= i && i <= je) || (js <= i + N && i + N <= je))) {
+          js = i + 1;
+          je = i + N - 1;
+        }
+        S.js[i] = (uint16_t)js;
+        S.je[i] = (uint16_t)je;
+      }
+      __syncthreads();
+    }
+    force_pass<NT, NS, PT>(J, mol, N, c, S, R.site_f);
+    calc_properties<NT>(J, mol, N, c, R.sc, S.red);
+    if (threadIdx.x == 0) store_sc(c, R.sc);
+    return;
+  }
+
+  // MODE_INTERVAL
+  const double xflag = R.sc[S_XFLAG];
+  if (xflag != 0.0) {
+    // REMD::ReplicaExchange for an exchanged replica
+    // (src/remd.cxx:526-536): ChangeTemperature, KillMomentum, SetPressure,
+    // CalcProperties, UpdateHamiltonian.
+    const double Tn = R.sc[S_TNEW];
+    const double coef = sqrt(Tn / c.T);
+    for (int i = threadIdx.x; i < N; i += NT) {
+      for (int d = 0; d < 3; d++) mol[(F_VX + d) * N + i] *= coef / c.s;
+      for (int d = 0; d < 4; d++) mol[(F_P0 + d) * N + i] *= coef / c.s;
+    }
+    c.Ps *= coef;
+    if (J.pmode == 1) c.Pv[0] *= coef * c.s;
+    if (J.pmode == 2) c.Pv[2] *= coef * c.s;
+    if (J.pmode == 3) c.Pv[0] *= coef * c.s;
+    c.T = Tn;
+    c.gkT = 6.0 * (double)N * Tn;
+    c.s = 1.0;
+    __syncthreads();
+    kill_momentum<NT>(J, mol, N, c, S.red, false);
+    c.P = R.sc[S_PNEW];
+    c.H0 = total_energy<NT>(J, mol, N, c, S.red);
+  }
+  if (J.tmode == 2) velocity_scaling<NT>(J, mol, N, c, S.red);
+  for (int st = 0; st < J.interval; ++st) {
+    if (J.tmode == 2) velocity_scaling<NT>(J, mol, N, c, S.red);
+    const bool do_sort = (st % J.sort_interval) == 0;
+    md_step<NT, NS, PT>(J, mol, mid, N, c, S, do_sort, R);
+  }
+  calc_properties<NT>(J, mol, N, c, R.sc, S.red);
+  if (threadIdx.x == 0) {
+    store_sc(c, R.sc);
+    R.sc[S_XFLAG] = 0.0;
+    R.sc[S_STEPS] += (double)J.interval;
+  }
+}
+
+template <int NT, int NS, typename PT>
+__global__ __launch_bounds__(NT) void interval_kernel(LaunchArgs a) {
+  extern __shared__ __align__(16) char smem[];
+  if ((int)blockIdx.x >= a.nrep) return;
+  const ReplicaDesc R = a.reps[blockIdx.x];
+  const JobParams &J = *R.job;
+  run_replica<NT, NS, PT>(a, R, J, smem);
+}
+
+// ---------------------------------------------------------------- b1
+// ForceCalculator::CalcWallForce, tube (src/force_calculator.cxx:117-161):
+// one thread per table node, 10000-point phi sum in FP64 with float sqrt as
+// the reference, in the reference's summation order.
+__global__ void wall_tube_energy_kernel(int nfwall, double R, double sgm06, double coefE, double *fw) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nfwall) return;
+  const double dr = R / (double)nfwall;
+  const int ninteg = 10000;
+  const double dphi = M_PI / (double)ninteg;
+  const double R02 = R * R;
+  const double r = (double)i * dr;
+  const double r02 = r * r;
+  double sum = 0.0;
+  for (int j = 0; j < ninteg; j++) {
+    const double phi = M_PI - (double)j * dphi;
+    const double rho_min = (double)sqrtf((float)(R02 + r02 - 2.0 * R * r * cos(phi)));
+    const double rho_mini = 1.0 / rho_min;
+    const double rho_min05i = rho_mini * rho_mini * rho_mini * rho_mini * rho_mini;
+    double tmp = 63.0 / 32.0 * sgm06 * rho_min05i * rho_min05i * rho_mini;
+    tmp -= 3.0 * rho_min05i;
+    sum += tmp;
+  }
+  fw[2 * i + 1] = coefE * sum * R * dphi;
+}
+
+__global__ void wall_tube_force_kernel(int nfwall, double R, double *fw) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double dr = R / (double)nfwall;
+  if (i >= 1 && i < nfwall - 1) {
+    const double r = (double)i * dr;
+    fw[2 * i] = -(fw[2 * (i + 1) + 1] - fw[2 * (i - 1) + 1]) / (2.0 * dr * r);
+  }
+}
+
+__global__ void wall_tube_edges_kernel(int nfwall, double *fw) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  fw[0] = fw[2];
+  fw[2 * (nfwall - 1)] = (fw[2 * (nfwall - 2)] - fw[2 * (nfwall - 3)]) * 2.0 + fw[2 * (nfwall - 3)];
+}
+
+// Slit wall (10-4 graphene sheets at z = +-H/2; see the oracle's contract).
+__device__ inline double w104_e(double h, double sgm, double eps, double rho) {
+  const double x = sgm / h, x2 = x * x, x4 = x2 * x2, x10 = x4 * x4 * x2;
+  return 2.0 * M_PI * rho * eps * sgm * sgm * (0.4 * x10 - x4);
+}
+__device__ inline double w104_dedh(double h, double sgm, double eps, double rho) {
+  const double x = sgm / h, x2 = x * x, x4 = x2 * x2, x10 = x4 * x4 * x2;
+  return 2.0 * M_PI * rho * eps * sgm * sgm * (-4.0 * x10 + 4.0 * x4) / h;
+}
+__global__ void wall_slit_kernel(int nfwall, double H, double sgm, double eps, double rho, double *fw) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nfwall) return;
+  const double dr = 0.5 * H / (double)nfwall;
+  const double z = (double)i * dr;
+  const double hd = 0.5 * H + z, hu = 0.5 * H - z;
+  fw[2 * i + 1] = w104_e(hd, sgm, eps, rho) + w104_e(hu, sgm, eps, rho);
+  const double dedz = w104_dedh(hd, sgm, eps, rho) - w104_dedh(hu, sgm, eps, rho);
+  fw[2 * i] = (i == 0) ? 0.0 : -dedz / z;
+}
+__global__ void wall_slit_edge_kernel(double *fw) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) fw[0] = fw[2];
+}
+
+// ----------------------------------------------------------------- c1
+// Standalone sort + window for the parity hook: one workgroup per row of
+// n keys (fractional coordinates in [0,1)).
+template <int NT>
+__global__ __launch_bounds__(NT) void sort_window_kernel(const double *keys, int n, double w, int *order,
+                                                         int *js_out, int *je_out) {
+  extern __shared__ __align__(16) char smem[];
+  const double *z = keys + (size_t)blockIdx.x * n;
+  uint64_t *keyA = (uint64_t *)smem;
+  uint64_t *keyB = keyA + n;
+  uint16_t *idxA = (uint16_t *)(keyB + n);
+  uint16_t *idxB = idxA + n;
+  int *cnt = (int *)(idxB + n + (n & 1));
+  for (int i = threadIdx.x; i < n; i += NT) {
+    keyA[i] = (uint64_t)__double_as_longlong(z[i] + 0.0);
+    idxA[i] = (uint16_t)i;
+  }
+  __syncthreads();
+  radix_sort<NT>(keyA, idxA, keyB, idxB, cnt, n);
+  double *zs = (double *)keyB;
+  for (int i = threadIdx.x; i < n; i += NT) zs[i] = z[idxA[i]];
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += NT) {
+    int js, je;
+    window_of(zs, n, i, w, js, je);
+    order[(size_t)blockIdx.x * n + i] = idxA[i];
+    js_out[(size_t)blockIdx.x * n + i] = js;
+    je_out[(size_t)blockIdx.x * n + i] = je;
+  }
+}
+
+// ----------------------------------------------------------------- e1
+// Metropolis over one pair list (REMD::ReplicaExchange,
+// src/remd.cxx:486-538 with REMDInfo::ExchangeAccepted/Rejected,
+// src/remdinfo.h:276-359) on device, fed uniforms drawn by the host in
+// pair-list order.  One thread per job; writes the new conditions and the
+// exchange flag into each replica's scalar block.
+struct ExchangeJob {
+  int ntemp, npress, pmode, ec_type;
+  int *index;         // condition slot -> replica
+  double *T, *P;      // per replica conditions (reduced)
+  const double *u;    // uniforms, pair-list order
+  double *const *sc;  // per replica scalar blocks
+  int *accepted;      // per pair, may be null
+};
+
+__global__ void exchange_kernel(const ExchangeJob *jobs, int njobs) {
+  const int jid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (jid >= njobs) return;
+  const ExchangeJob &X = jobs[jid];
+  const int dt = X.ntemp, dp = X.npress;
+  const int nrep = dt * dp;
+  for (int r = 0; r < nrep; r++) X.sc[r][S_XFLAG] = 0.0;
+  int k = 0;
+  auto visit = [&](int a, int b) {
+    const int ind1 = X.index[a], ind2 = X.index[b];
+    const double beta1 = 1. / X.T[ind1], beta2 = 1. / X.T[ind2];
+    const double pot1 = X.sc[ind1][S_POT], pot2 = X.sc[ind2][S_POT];
+    double delta = (beta2 - beta1) * (pot1 - pot2);
+    if (X.pmode > 0) {
+      const double v1 = X.sc[ind1][S_VOL], v2 = X.sc[ind2][S_VOL];
+      delta += (beta2 * X.P[ind2] - beta1 * X.P[ind1]) * (v1 - v2);
+    }
+    const int acc = exp(-delta) > X.u[k];
+    if (X.accepted) X.accepted[k] = acc;
+    k++;
+    if (acc) {
+      X.sc[ind1][S_XFLAG] = 1.0;
+      X.sc[ind2][S_XFLAG] = 1.0;
+      double t = X.T[ind1]; X.T[ind1] = X.T[ind2]; X.T[ind2] = t;
+      t = X.P[ind1]; X.P[ind1] = X.P[ind2]; X.P[ind2] = t;
+      X.index[a] = ind2;
+      X.index[b] = ind1;
+    } else {
+      X.sc[ind1][S_XFLAG] = 0.0;
+      X.sc[ind2][S_XFLAG] = 0.0;
+    }
+  };
+  if (X.ec_type == 0) {
+    for (int j = 0; j < dp; ++j)
+      for (int i = 0; i < dt / 2; ++i) visit(2 * i + j * dt, 2 * i + 1 + j * dt);
+  } else if (X.ec_type == 1) {
+    for (int j = 0; j < dp / 2; ++j)
+      for (int i = 0; i < dt; ++i) visit(i + 2 * j * dt, i + (2 * j + 1) * dt);
+  } else if (X.ec_type == 2) {
+    for (int j = 0; j < dp; ++j)
+      for (int i = 0; i < dt / 2 - (dt + 1) % 2; ++i) visit(2 * i + j * dt + 1, 2 * i + 1 + j * dt + 1);
+  } else {
+    for (int j = 0; j < dp / 2 - (dp + 1) % 2; ++j)
+      for (int i = 0; i < dt; ++i) visit(i + (2 * j + 1) * dt, i + (2 * j + 2) * dt);
+  }
+  for (int r = 0; r < nrep; r++) {
+    X.sc[r][S_TNEW] = X.T[r];
+    X.sc[r][S_PNEW] = X.P[r];
+  }
+}
+
+// ------------------------------------------------------------ launchers
+// Kernel class of a replica: (block threads, sites, pair precision).
+template <int NT>
+static hipError_t launch_nt(const LaunchArgs &a, int nsite, int prec, int lds, hipStream_t st) {
+  const dim3 g(a.nrep), b(NT);
+  const int key = nsite * 2 + prec;
+  switch (key) {
+    case 6: hipLaunchKernelGGL((interval_kernel<NT, 3, float>), g, b, lds, st, a); break;
+    case 8: hipLaunchKernelGGL((interval_kernel<NT, 4, float>), g, b, lds, st, a); break;
+    case 10: hipLaunchKernelGGL((interval_kernel<NT, 5, float>), g, b, lds, st, a); break;
+    case 7: hipLaunchKernelGGL((interval_kernel<NT, 3, double>), g, b, lds, st, a); break;
+    case 9: hipLaunchKernelGGL((interval_kernel<NT, 4, double>), g, b, lds, st, a); break;
+    case 11: hipLaunchKernelGGL((interval_kernel<NT, 5, double>), g, b, lds, st, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_interval(const LaunchArgs &a, int nt, int nsite, int prec, int lds, hipStream_t st) {
+  if (a.nrep <= 0) return hipSuccess;
+  if (nt == 256) return launch_nt<256>(a, nsite, prec, lds, st);
+  if (nt == 512) return launch_nt<512>(a, nsite, prec, lds, st);
+  if (nt == 128) return launch_nt<128>(a, nsite, prec, lds, st);
+  return hipErrorInvalidValue;
+}
+
+template <int NT>
+static hipError_t set_lds_nt(int bytes) {
+  hipError_t e = hipSuccess;
+#define REMD_SETLDS(NS_, PT_)                                                                              \
+  if (e == hipSuccess)                                                                                      \
+    e = hipFuncSetAttribute((const void *)interval_kernel<NT, NS_, PT_>,                                    \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  REMD_SETLDS(3, float)
+  REMD_SETLDS(4, float)
+  REMD_SETLDS(5, float)
+  REMD_SETLDS(3, double)
+  REMD_SETLDS(4, double)
+  REMD_SETLDS(5, double)
+#undef REMD_SETLDS
+  return e;
+}
+
+hipError_t set_kernel_lds_limit(int bytes) {
+  hipError_t e = set_lds_nt<256>(bytes);
+  if (e == hipSuccess) e = set_lds_nt<512>(bytes);
+  if (e == hipSuccess) e = set_lds_nt<128>(bytes);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void *)sort_window_kernel<256>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            bytes);
+  return e;
+}
+
+hipError_t launch_wall_tube(int nfwall, double R, double sgm06, double coefE, double *fw, hipStream_t st) {
+  const int nb = (nfwall + 255) / 256;
+  hipLaunchKernelGGL(wall_tube_energy_kernel, dim3(nb), dim3(256), 0, st, nfwall, R, sgm06, coefE, fw);
+  hipLaunchKernelGGL(wall_tube_force_kernel, dim3(nb), dim3(256), 0, st, nfwall, R, fw);
+  hipLaunchKernelGGL(wall_tube_edges_kernel, dim3(1), dim3(64), 0, st, nfwall, fw);
+  return hipGetLastError();
+}
+
+hipError_t launch_wall_slit(int nfwall, double H, double sgm, double eps, double rho, double *fw,
+                            hipStream_t st) {
+  const int nb = (nfwall + 255) / 256;
+  hipLaunchKernelGGL(wall_slit_kernel, dim3(nb), dim3(256), 0, st, nfwall, H, sgm, eps, rho, fw);
+  hipLaunchKernelGGL(wall_slit_edge_kernel, dim3(1), dim3(64), 0, st, fw);
+  return hipGetLastError();
+}
+
+hipError_t launch_sort_window(const double *keys, int nrow, int n, double w, int *order, int *js, int *je,
+                              hipStream_t st) {
+  const int lds = 20 * n + 8 + (16 * 64 + 16) * 4;
+  hipLaunchKernelGGL(sort_window_kernel<256>, dim3(nrow), dim3(256), lds, st, keys, n, w, order, js, je);
+  return hipGetLastError();
+}
+
+hipError_t launch_exchange(const ExchangeJob *jobs, int njobs, hipStream_t st) {
+  hipLaunchKernelGGL(exchange_kernel, dim3((njobs + 63) / 64), dim3(64), 0, st, jobs, njobs);
+  return hipGetLastError();
+}
+
+}  // namespace remd

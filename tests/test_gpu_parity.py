@@ -1,0 +1,202 @@
+"""HIP path vs the oracle and the reference fixtures (needs the GPU).
+
+Tolerances (stated per quantity):
+  * integer/index work (sort order, windows, exchange decisions): bit-exact;
+  * step-0 forces and torques, FP32 pair math + FP64 accumulation:
+    relative RMS <= 1e-5 against the FP64 oracle / reference FP64 build;
+  * the same with FP64 pair math: relative RMS <= 1e-11;
+  * energies at step 0: |dE| <= 1e-5 * sum|E| (mixed), 1e-10 (FP64);
+  * one 100-step interval from the reference fixture: FP64 pair math
+    <= 1e-7 relative on positions / momenta (chaotic growth of 1e-16
+    summation-order differences), mixed <= 1e-3.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as po
+import remd_amd as ra
+from remd_amd import inputs as inp
+from helpers import PROP_KEYS, by_id, fixture_system, oracle_for, rel_rms
+
+pytestmark = pytest.mark.gpu
+
+
+def _props(d, key):
+    return dict(zip(PROP_KEYS, d[key]))
+
+
+# ------------------------------------------------------------------ b1
+@pytest.mark.parametrize("R,nfw", [(7.0, 1000), (7.4, 4096), (12.0, 257)])
+def test_wall_table_tube_matches_oracle(R, nfw):
+    g = ra.wall_table(1, nfw, R, 3.277, 0.3886, 0.38016)
+    o = po.wall_table_tube(nfw, R, 3.277, 0.3886, 0.38016)
+    assert np.allclose(g, o, rtol=1e-12, atol=1e-12 * np.max(np.abs(o)))
+
+
+def test_wall_table_tube_matches_reference_fixture(golden):
+    d = golden("ref_wall_tables.npz")
+    g = ra.wall_table(1, 1000, 7.0, 3.277, 0.3886, 0.38016)
+    ref = d["cfg2"]
+    assert np.allclose(g, ref, rtol=1e-12, atol=1e-12 * np.max(np.abs(ref)))
+
+
+def test_wall_table_slit_matches_oracle():
+    g = ra.wall_table(2, 1000, 10.0, 3.277, 0.3886, 0.38016)
+    o = po.wall_table_slit(1000, 10.0, 3.277, 0.3886, 0.38016)
+    assert np.allclose(g, o, rtol=1e-12, atol=1e-12 * np.max(np.abs(o)))
+
+
+# ------------------------------------------------------------------ c1
+@pytest.mark.parametrize("n,w,seed", [(144, 0.19, 0), (150, 0.32, 1), (512, 0.24, 2), (2048, 0.1, 3),
+                                      (37, 0.45, 4), (256, 0.482, 5)])
+def test_sort_window_bitexact(n, w, seed):
+    rng = np.random.RandomState(seed)
+    keys = rng.random_sample((3, n))
+    keys[1, : n // 3] = keys[1, 0]  # ties
+    keys[2] = np.round(keys[2] * 7) / 7.0  # lattice layers with many ties
+    keys[2] = np.mod(keys[2], 1.0)
+    order, js, je = ra.sort_window(keys, w)
+    for row in range(3):
+        o2, js2, je2 = po.sort_window(keys[row], w)
+        assert np.array_equal(order[row], o2)
+        assert np.array_equal(js[row], js2)
+        assert np.array_equal(je[row], je2)
+
+
+# ------------------------------------------------------------- a1..a5, b2
+@pytest.mark.parametrize("prec,tol_f,tol_e", [(0, 1e-5, 1e-5), (1, 1e-11, 1e-10)])
+def test_step0_forces_reference_fixture(golden, prec, tol_f, tol_e):
+    d = golden("ref_tube_chk.npz")
+    sysm = fixture_system(d, "f64")
+    g = ra.REMDGPU(sysm, precision=prec)
+    s = g.state(0)
+    ref = d["f64_init_state"]
+    ids_ref = d["f64_init_ids"]
+    assert np.array_equal(s["id"], ids_ref)  # same sort order
+    f_ref, n_ref = ref[:, 14:17], ref[:, 17:21]
+    ef = rel_rms(s["f"], f_ref)
+    en = rel_rms(s["n"], n_ref)
+    print(f"prec={prec} force rel-RMS {ef:.3e} torque rel-RMS {en:.3e}")
+    assert ef <= tol_f and en <= tol_f
+    pr = _props(d, "f64_init_props")
+    av = g.averages(0)
+    scale = abs(pr["lj"]) + abs(pr["clmb"]) + abs(pr["wall"])
+    for k, idx in (("lj", 0), ("clmb", 1), ("wall", 2)):
+        print(k, av[idx], pr[k], av[idx] - pr[k])
+        assert abs(av[idx] - pr[k]) <= tol_e * scale
+    g.close()
+
+
+@pytest.mark.parametrize("model,conf", [("TIP3P", 1), ("TIP4P", 1), ("TIP5P", 1), ("TIP4P", 0), ("TIP5P", 2),
+                                        ("TIP3P", 2)])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_step0_forces_vs_oracle(model, conf, prec):
+    if conf == 0:
+        sysm = inp.config_bulk(seed=11, ntemp=2, model=model)
+    elif conf == 1:
+        sysm = inp.config_tube(seed=12, nmol=120, ntemp=2, npress=1, model=model)
+    else:
+        sysm = inp.config_slit(seed=13, nmol=200, ntemp=2, npress=1, model=model, Lxy=30.0)
+    g = ra.REMDGPU(sysm, precision=prec)
+    g.set_debug(True)
+    g.compute_forces(do_sort=True, margin=0.0)
+    tol = 1e-5 if prec == 0 else 1e-11
+    for rep in range(sysm.nreplica):
+        o = oracle_for(sysm, rep)
+        o.calc_force(True, 0.0)
+        so = o.state()
+        sg = g.state(rep)
+        assert np.array_equal(sg["id"], so["id"])
+        js, je = g.window(rep)
+        assert np.array_equal(js, so["jstart"]) and np.array_equal(je, so["jend"])
+        ef = rel_rms(sg["f"], so["f"])
+        en = rel_rms(sg["n"], so["n"])
+        esf = rel_rms(g.site_forces(rep), so["site_f"])
+        print(model, conf, prec, rep, f"F {ef:.2e} N {en:.2e} site {esf:.2e}")
+        assert ef <= tol and en <= tol and esf <= tol
+        av = g.averages(rep)
+        sc = o.scalars()
+        scale = abs(sc["lj"]) + abs(sc["clmb"]) + abs(sc["wall"]) + 1.0
+        for k, idx in (("lj", 0), ("clmb", 1), ("wall", 2)):
+            assert abs(av[idx] - sc[k]) <= 10 * tol * scale, (k, av[idx], sc[k])
+    g.close()
+
+
+# -------------------------------------------------------------- d1 (+a-c)
+@pytest.mark.parametrize("prec,tol", [(1, 1e-7), (0, 1e-3)])
+def test_interval_reference_fixture(golden, prec, tol):
+    d = golden("ref_tube_chk.npz")
+    sysm = fixture_system(d, "f64")
+    g = ra.REMDGPU(sysm, precision=prec)
+    g.execute_steps(sysm.T, sysm.P)
+    s = g.state(0)
+    ref = d["f64_int_state"]
+    ids = d["f64_int_ids"]
+    for name, sl in (("r", slice(0, 3)), ("v", slice(3, 6)), ("q", slice(6, 10)), ("p", slice(10, 14))):
+        a = by_id(s[name], s["id"])
+        b = by_id(ref[:, sl], ids)
+        if name == "r":  # fractional, compare the periodic difference
+            dd = a - b
+            dd -= np.rint(dd)
+            err = float(np.max(np.abs(dd)))
+        else:
+            err = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+        print(f"prec={prec} {name} max rel err {err:.3e}")
+        assert err <= tol
+    pr = _props(d, "f64_int_props")
+    av = g.averages(0)
+    for k, idx in (("lj", 0), ("clmb", 1), ("wall", 2), ("tot", 11)):
+        print(k, av[idx], pr[k])
+    assert abs(av[11] - pr["tot"]) <= 1e3 * tol * abs(pr["tot"]) + 1e-6
+    g.close()
+
+
+# ------------------------------------------------------------------ e1
+def test_exchange_device_matches_oracle():
+    sysm = inp.config_tube(seed=21, nmol=64, Lz=30.0, ntemp=4, npress=2, nfwall=500)
+    g = ra.REMDGPU(sysm, precision=0)
+    pot, vol = g.pot_vol()
+    rng = np.random.RandomState(5)
+    index = np.arange(8, dtype=np.int32)
+    T = sysm.T.copy()
+    P = sysm.P.copy()
+    for step in range(8):
+        u = rng.random_sample(8)
+        i2, T2, P2, acc2, _ = po.replica_exchange(4, 2, step, 2, pot, vol, index, T, P, u)
+        i1, T1, P1, acc1 = g.exchange_device(4, 2, step, index, T, P, u)
+        assert np.array_equal(i1, i2) and np.array_equal(T1, T2) and np.array_equal(P1, P2)
+        assert np.array_equal(acc1, acc2[: len(acc1)])
+        index, T, P = i1, T1, P1
+    g.close()
+
+
+# ------------------------------------------------------------- batching
+def test_batch_equals_alone_and_plain():
+    a = inp.config_tube(seed=31, nmol=60, Lz=26.0, ntemp=2, npress=1, nfwall=500)
+    b = inp.config_tube(seed=32, nmol=48, Lz=24.0, ntemp=2, npress=1, model="TIP3P", nfwall=500)
+    for s in (a, b):
+        s.interval = 20
+    alone = {}
+    for name, s in (("a", a), ("b", b)):
+        g = ra.REMDGPU(s)
+        g.execute_steps(s.T, s.P)
+        alone[name] = (g.all_averages(), g.state(0)["r"])
+        g.close()
+    # plain (one replica per launch) gives the same bits
+    g = ra.REMDGPU(a)
+    g.execute_steps(a.T, a.P, plain=True)
+    assert np.array_equal(g.all_averages(), alone["a"][0])
+    g.close()
+    ga, gb = ra.REMDGPU(a), ra.REMDGPU(b)
+    bt = ra.Batch()
+    bt.add(ga)
+    bt.add(gb)
+    ga.set_conditions(a.T, a.P)
+    gb.set_conditions(b.T, b.P)
+    bt.execute()
+    assert np.array_equal(ga.all_averages(), alone["a"][0])
+    assert np.array_equal(gb.all_averages(), alone["b"][0])
+    assert np.array_equal(ga.state(0)["r"], alone["a"][1])
+    bt.close()
+    ga.close()
+    gb.close()
