@@ -1,0 +1,212 @@
+#!/usr/bin/env python3
+"""Benchmark: replica-steps/s of the REMD interval path on one MI355X.
+
+Workload (the plain-vs-GPU drop-in configuration): nanotube (confined=1),
+144 TIP4P per replica, R = 7 A, Lz = 50 A, 1D Andersen barostat along z,
+Nose-Poincare thermostat, 16 T (200-360 K geometric) x 4 P (0.1-200 MPa
+log) = 64 replicas, rcut 9 A, rswitch 8 A, dt 2 fs, 100 steps per exchange
+interval.  Synthetic seeded inputs, equilibrated on the device first.
+
+A "step" of this benchmark is one exchange interval of the whole job
+(64 replicas x 100 MD steps, one kernel launch) followed by the host
+Metropolis exchange; `value` = replica-MD-steps per second over the timed
+steps, inputs resident in HBM.
+
+    python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-plain]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3  # MI355X dense FP32 vector peak
+
+
+def cpu_baseline(system, T, P, budget_s=30.0):
+    """Time the reference's own FP32 CPU build (oracle/_ref, makefile.gnu
+    flags, one thread) on the same relaxed replicas, one interval each;
+    falls back to the oracle's FP32 port where _ref was not built."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import tempfile
+
+    import pyoracle as po
+    from remd_amd import inputs as inp
+
+    kind = "reference" if po.ref_available() else "port"
+    nrep = system.nreplica
+    steps = system.interval
+    done = 0
+    elapsed = 0.0
+    mt_text = inp.MOLTYPE[system.model]
+    with tempfile.TemporaryDirectory() as td:
+        mtl = os.path.join(td, "moltype.txt")
+        with open(mtl, "w") as f:
+            f.write(mt_text)
+        fw = None if kind == "reference" else po.wall_table_tube(system.nfwall, system.wall_length,
+                                                                system.sgm_wall, system.eps_wall,
+                                                                system.rho_wall)
+        reflib = po.RefLib(True) if kind == "reference" else None
+        for k in range(nrep):
+            st = system.states[k]
+            desc = system.desc()
+            if kind == "reference":
+                rs = po.RefSystem(reflib, mtl, desc, st.r, st.v, st.q, st.p, st.ids, st.scal(), T[k], P[k])
+                rs.quiet(reflib.L.ref_initialize_property)
+                t0 = time.perf_counter()
+                rs.quiet(reflib.L.ref_execute_steps)
+                elapsed += time.perf_counter() - t0
+                del rs
+            else:
+                o = po.OracleSystem(desc, po.parse_moltype(mt_text), st.r, st.v, st.q, st.p, st.scal(), T[k], P[k],
+                                    fp32=True, fwall=fw)
+                o.initialize_property()
+                t0 = time.perf_counter()
+                o.execute_steps(steps)
+                elapsed += time.perf_counter() - t0
+            done += 1
+            if elapsed > budget_s:
+                break
+    value = done * steps / elapsed
+    sample = (f"{done} of {nrep} replicas x 1 interval ({steps} steps), makefile.gnu FP32 build, 1 thread"
+              if kind == "reference" else
+              f"{done} of {nrep} replicas x 1 interval ({steps} steps), oracle FP32 port, 1 thread")
+    return dict(value=value, unit="replica-steps/s", cores=1, kind=kind, sample=sample,
+                seconds=round(elapsed, 3))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20, help="timed exchange intervals")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--plain-steps", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: replicas shard across GPUs as whole jobs/ladders with no "
+              "per-step communication, so only --gpus 1 is run here.", file=sys.stderr)
+        sys.exit(2)
+
+    import remd_amd as ra
+    from remd_amd import inputs, workloads
+
+    system = inputs.config_tube(seed=2)
+    system = workloads.relax(system)
+    nrep, interval = system.nreplica, system.interval
+    T = system.T.copy()
+    P = system.P.copy()
+    job = ra.REMDGPU(system, precision=0)
+    shape = job.launch_shape()
+
+    index = np.arange(nrep, dtype=np.int32)
+    rng = ra.GlibcRand(1)
+    step_no = 0
+
+    def one_interval(plain=False):
+        nonlocal step_no
+        job.execute_steps(T, P, plain=plain)
+        pot, vol = job.pot_vol()
+        ra.exchange_host(system.ntemp, system.npress, step_no, system.pconstant, pot, vol, index, T, P, rng)
+        step_no += 1
+
+    for _ in range(args.warmup):
+        one_interval()
+
+    # algorithmic work: ordered in-cutoff pairs at the interval-start snapshot
+    flops_per_launch = workloads.algorithmic_flops(job, interval)
+
+    lat = []
+    kms = []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        one_interval()
+        lat.append(time.perf_counter() - t0)
+        kms.append(job.last_kernel_ms())
+    wall = time.perf_counter() - t_start
+    value = nrep * interval * args.steps / wall
+    ms_per_step = wall / args.steps * 1e3
+    kernel_ms = float(np.mean(kms))
+
+    plain = None
+    if not args.no_plain:
+        t0 = time.perf_counter()
+        for _ in range(args.plain_steps):
+            one_interval(plain=True)
+        pw = time.perf_counter() - t0
+        plain = nrep * interval * args.plain_steps / pw
+
+    # force relative error of the FP32-pair/FP64-accumulate kernels against
+    # the same kernels in FP64, on the current state of replica 0
+    snap = inputs.System(**{k: getattr(system, k) for k in (
+        "model", "nmol", "tconstant", "pconstant", "confined", "dt_fs", "rcut", "rswitch", "interval", "ntemp",
+        "npress", "T", "P", "wall_length", "nfwall", "sgm_wall", "eps_wall", "rho_wall")})
+    snap.ntemp, snap.npress = 1, 1
+    st = job.state(0)
+    snap.T, snap.P = T[:1].copy(), P[:1].copy()
+    snap.states = [inputs.ReplicaState(r=st["r"], v=st["v"], q=st["q"], p=st["p"], s=st["scal"][0],
+                                       Ps=st["scal"][1], Q=st["scal"][2], Pv=tuple(st["scal"][3:6]),
+                                       W=st["scal"][6], L=tuple(st["scal"][7:10]), ids=st["id"])]
+    fm = ra.REMDGPU(snap, precision=0)
+    fd = ra.REMDGPU(snap, precision=1)
+    f_mixed, f_fp64 = fm.state(0)["f"], fd.state(0)["f"]
+    force_err = float(np.sqrt(np.sum((f_mixed - f_fp64) ** 2) / np.sum(f_fp64 ** 2)))
+    fm.close()
+    fd.close()
+
+    achieved = flops_per_launch / (kernel_ms * 1e-3) / 1e12
+    out = {
+        "metric": "replica-steps/s batched vs plain at <=1e-4 force rel err; p99 job latency",
+        "value": round(value, 1),
+        "unit": "replica-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pair math, f64 accumulation and integrator",
+        "data": "synthetic (seeded placement, device-equilibrated)",
+        "config": {
+            "workload": "nanotube TIP4P, 144 mol/replica, R 7 A, Lz 50 A, 16T x 4P = 64 replicas, "
+                        "rcut 9 A, dt 2 fs, 100 steps/interval, NP thermostat + 1D barostat",
+            "replicas": nrep, "nmol": system.nmol, "steps_per_interval": interval,
+            "launch": {"block_threads": shape["block_threads"], "chunks": shape["chunks"],
+                       "lds_bytes": shape["lds_bytes"]},
+        },
+        "plain_value": None if plain is None else round(plain, 1),
+        "batched_over_plain": None if plain is None else round(value / plain, 2),
+        "force_rel_err_vs_fp64": force_err,
+        "ns_per_day_per_replica": round(workloads.ns_per_day(value, nrep, system.dt_fs), 3),
+        "interval_latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                                "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
+        "roofline": {
+            "bound": "valu",
+            "kernel": "interval_kernel<256,4,float>",
+            "achieved": round(achieved, 4),
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP32_TFLOPS, 5),
+            "traffic": None,
+            "flops_per_launch": flops_per_launch,
+            "kernel_ms": round(kernel_ms, 4),
+        },
+    }
+    job.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(system, system.T, system.P)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

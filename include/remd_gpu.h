@@ -173,6 +173,17 @@ typedef struct { int r[34]; int k; } remd_rng;
 void remd_rng_seed(remd_rng *s, unsigned seed);
 int remd_rng_next(remd_rng *s);
 
+/* REMD::ReplicaExchange (src/remd.cxx:486-538) on the host for one job:
+ * pair list step%4 (src/remdinfo.cxx:54-108), Metropolis with one
+ * rng draw per pair in list order, REMDInfo::ExchangeAccepted/Rejected
+ * (src/remdinfo.h:276-359) on index/T/P (reduced; T, P indexed by
+ * replica, index by condition slot).  is_exchanged[rep] (may be NULL) and
+ * accepted[pair] (may be NULL) report the outcome; returns the number of
+ * accepted swaps or a negative error. */
+int remd_exchange_host(int ntemp, int npress, long step, int pconstant, const double *pot,
+                       const double *vol, int *index, double *T, double *P,
+                       unsigned char *is_exchanged, remd_rng *rng, int *accepted, int *npairs);
+
 #ifdef __cplusplus
 }
 #endif

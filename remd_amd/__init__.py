@@ -15,7 +15,7 @@ import numpy as np
 from . import inputs
 
 __all__ = ["REMDGPU", "Batch", "lib", "desc_for", "GlibcRand", "wall_table", "sort_window",
-           "AVERAGE_NAMES", "inputs"]
+           "AVERAGE_NAMES", "inputs", "exchange_host"]
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(PKG, "libremd_gpu.so")
@@ -104,6 +104,9 @@ def lib():
     L.remd_rng_next.argtypes = [C.POINTER(Rng)]
     L.remd_rng_next.restype = C.c_int
     L.remd_gpu_abi_version.restype = C.c_int
+    L.remd_exchange_host.argtypes = [C.c_int, C.c_int, C.c_long, C.c_int, _dp, _dp, _ip, _dp, _dp,
+                                     C.POINTER(C.c_ubyte), C.POINTER(Rng), _ip, _ip]
+    L.remd_exchange_host.restype = C.c_int
     _lib = L
     return L
 
@@ -331,6 +334,24 @@ class GlibcRand:
 
     def uniform(self) -> float:
         return self.rand() / 2147483647.0
+
+
+def exchange_host(ntemp, npress, step, pconstant, pot, vol, index, T, P, rng: "GlibcRand"):
+    """REMD::ReplicaExchange on the host (in place on index/T/P); returns
+    (n_accepted, accepted per pair, is_exchanged per replica)."""
+    n = ntemp * npress
+    pot = np.ascontiguousarray(pot, np.float64)
+    vol = np.ascontiguousarray(vol, np.float64)
+    assert index.dtype == np.int32 and T.dtype == np.float64 and P.dtype == np.float64
+    isx = np.zeros(n, np.uint8)
+    acc = np.zeros(n + 1, np.int32)
+    npairs = C.c_int()
+    rc = lib().remd_exchange_host(int(ntemp), int(npress), int(step), int(pconstant), _dp_of(pot), _dp_of(vol),
+                                  _ip_of(index), _dp_of(T), _dp_of(P), isx.ctypes.data_as(C.POINTER(C.c_ubyte)),
+                                  C.byref(rng.s), _ip_of(acc), C.byref(npairs))
+    if rc < 0:
+        _check(rc)
+    return rc, acc[: npairs.value], isx
 
 
 def wall_table(confined, nfwall, R, sgm, eps, rho) -> np.ndarray:

@@ -985,3 +985,51 @@ extern "C" int remd_rng_next(remd_rng *s) {
   s->k = (k + 1) % 34;
   return (int)(v >> 1);
 }
+
+// ------------------------------------------------------- host exchange
+extern "C" int remd_exchange_host(int dt, int dp, long step, int pconstant, const double *pot, const double *vol,
+                                  int *index, double *T, double *P, unsigned char *isx, remd_rng *rng,
+                                  int *accepted, int *npairs) {
+  if (dt < 1 || dp < 1 || !pot || !vol || !index || !T || !P || !rng) return fail(REMD_EINVAL, "bad argument");
+  const int n = dt * dp;
+  for (int k = 0; k < n; k++)
+    if (index[k] < 0 || index[k] >= n) return fail(REMD_EINVAL, "index out of range");
+  if (isx)
+    for (int r = 0; r < n; r++) isx[r] = 0;
+  int k = 0, nacc = 0;
+  auto visit = [&](int a, int b) {
+    const int ind1 = index[a], ind2 = index[b];
+    const double beta1 = 1. / T[ind1], beta2 = 1. / T[ind2];
+    double delta = (beta2 - beta1) * (pot[ind1] - pot[ind2]);
+    if (pconstant > 0) delta += (beta2 * P[ind2] - beta1 * P[ind1]) * (vol[ind1] - vol[ind2]);
+    const double u = (double)remd_rng_next(rng) / (double)2147483647;  // rand()/RAND_MAX
+    const int acc = exp(-delta) > u;
+    if (accepted) accepted[k] = acc;
+    k++;
+    if (acc) {
+      if (isx) isx[ind1] = isx[ind2] = 1;
+      std::swap(T[ind1], T[ind2]);
+      std::swap(P[ind1], P[ind2]);
+      std::swap(index[a], index[b]);
+      nacc++;
+    } else if (isx) {
+      isx[ind1] = isx[ind2] = 0;
+    }
+  };
+  const int ec = (int)(step % 4);
+  if (ec == 0) {
+    for (int j = 0; j < dp; ++j)
+      for (int i = 0; i < dt / 2; ++i) visit(2 * i + j * dt, 2 * i + 1 + j * dt);
+  } else if (ec == 1) {
+    for (int j = 0; j < dp / 2; ++j)
+      for (int i = 0; i < dt; ++i) visit(i + 2 * j * dt, i + (2 * j + 1) * dt);
+  } else if (ec == 2) {
+    for (int j = 0; j < dp; ++j)
+      for (int i = 0; i < dt / 2 - (dt + 1) % 2; ++i) visit(2 * i + j * dt + 1, 2 * i + 1 + j * dt + 1);
+  } else {
+    for (int j = 0; j < dp / 2 - (dp + 1) % 2; ++j)
+      for (int i = 0; i < dt; ++i) visit(i + (2 * j + 1) * dt, i + (2 * j + 2) * dt);
+  }
+  if (npairs) *npairs = k;
+  return nacc;
+}
