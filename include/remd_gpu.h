@@ -121,10 +121,15 @@ int remd_gpu_compute_forces(remd_gpu *g, int do_sort, double margin);
 /* Keep per-site forces and the window of every force pass (debug buffers). */
 int remd_gpu_set_debug(remd_gpu *g, int on);
 int remd_gpu_get_site_forces(remd_gpu *g, int rep, double *af /* nmol*nsite*3 */);
-int remd_gpu_get_window(remd_gpu *g, int rep, int *js, int *je);
+/* Window of the last sort: the reference's js/je (src/molecules.h:77-78)
+ * and full[i] = 1 where the window is the whole ring (may be NULL). */
+int remd_gpu_get_window(remd_gpu *g, int rep, int *js, int *je, int *full);
 /* Average duration (ms) of the interval kernel over the last
  * remd_gpu_execute_steps call, from HIP events on the job's stream. */
 int remd_gpu_last_kernel_ms(remd_gpu *g, double *ms);
+/* Phase timers of the last interval (REMD_PHASE_TIMERS builds; zeros
+ * otherwise): out[8] in 100 MHz ticks accumulated over the job so far. */
+int remd_gpu_phase_timers(remd_gpu *g, int rep, double out[8]);
 /* Chosen launch shape. */
 int remd_gpu_launch_shape(remd_gpu *g, int *block_threads, int *chunks, int *lds_bytes);
 /* Measure each (block_threads, chunks) candidate with timed force passes and

@@ -66,6 +66,7 @@ class System(C.Structure):
         ("ham", C.c_double), ("Tk", C.c_double), ("prs", C.c_double * 4),
         ("jstart", _ip), ("jend", _ip),
         ("sort_axis", C.c_int),
+        ("wfull", _ip),
     ]
 
 
@@ -195,6 +196,7 @@ class OracleSystem:
             dtype=np.float64)[: 3 * self.sys.mt.nsite * self.n].reshape(self.n, self.sys.mt.nsite, 3).copy()
         out["jstart"] = self._arr("jstart", 1, np.int32)[:, 0].copy()
         out["jend"] = self._arr("jend", 1, np.int32)[:, 0].copy()
+        out["wfull"] = self._arr("wfull", 1, np.int32)[:, 0].copy()
         return out
 
     def scalars(self):

@@ -142,6 +142,7 @@ int orc_system_alloc(orc_system *s, int nmol) {
   s->id = calloc((size_t)nmol, sizeof(int));
   s->jstart = calloc((size_t)nmol, sizeof(int));
   s->jend = calloc((size_t)nmol, sizeof(int));
+  s->wfull = calloc((size_t)nmol, sizeof(int));
   s->sort_interval = 10;
   s->margin = 0.5;
   s->sort_axis = 2;
@@ -151,12 +152,13 @@ int orc_system_alloc(orc_system *s, int nmol) {
     s->id[i] = i;
   }
   return (s->r && s->v && s->f && s->q && s->p && s->n && s->site_f && s->id &&
-          s->jstart && s->jend)
+          s->jstart && s->jend && s->wfull)
              ? 0
              : -1;
 }
 
 void orc_system_free(orc_system *s) {
+  free(s->wfull);
   free(s->r);
   free(s->v);
   free(s->f);
@@ -234,6 +236,18 @@ void orc_window(const double *z, int n, double w, int *js, int *je) {
   }
 }
 
+void orc_window_full(const double *z, int n, double w, const int *js,
+                     const int *je, int *full) {
+  for (int i = 0; i < n; i++) {
+    const int a = js[i], b = je[i];
+    /* a inside the upward arc, b inside the downward arc, with the scan's
+     * own bounds (z_i + w, minus 1 after a wrap; z_i - w, plus 1) */
+    const int a_up = (a > i) ? (z[a] <= z[i] + w) : (z[a] <= (z[i] + w) - 1.0);
+    const int b_dn = (b < i) ? (z[b] >= z[i] - w) : (z[b] >= (z[i] - w) + 1.0);
+    full[i] = (a == i || b == i || a_up || b_dn) ? 1 : 0;
+  }
+}
+
 void orc_sort_window(const double *key, int n, double w, int *order, int *js,
                      int *je) {
   int *tmp = malloc(sizeof(int) * (size_t)n);
@@ -273,6 +287,7 @@ static void sort_molecules(orc_system *s, double margin) {
   /* window on the sorted coordinate (src/molecules.cxx:556-583) */
   for (int i = 0; i < n; i++) key[i] = s->r[3 * i + ax];
   orc_window(key, n, (s->rcut + margin) / s->L[ax], s->jstart, s->jend);
+  orc_window_full(key, n, (s->rcut + margin) / s->L[ax], s->jstart, s->jend, s->wfull);
   free(key);
   free(order);
   free(tmp);

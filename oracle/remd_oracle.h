@@ -75,6 +75,7 @@ typedef struct {
   /* neighbour window (src/molecules.h:77-78) */
   int *jstart, *jend;
   int sort_axis;            /* 2 (z) for bulk/tube, 0 (x) for the slit */
+  int *wfull;               /* contract: 1 where the window is the whole ring */
 } orc_system;
 
 int  orc_system_alloc(orc_system *s, int nmol);
@@ -90,6 +91,13 @@ void orc_sort_window(const double *key, int n, double w,
                      int *order, int *js, int *je);
 /* Window only, on already sorted keys. */
 void orc_window(const double *sorted_key, int n, double w, int *js, int *je);
+/* Contract (no reference counterpart): full[i] = 1 when molecule i's
+ * window is the whole ring, i.e. the scan wrapped onto i itself, or the
+ * first key outside on one side lies inside the other side's arc (every
+ * key is within w of key i in one direction or the other).  The
+ * reference then visits only the js..je range, which misses pairs. */
+void orc_window_full(const double *sorted_key, int n, double w, const int *js,
+                     const int *je, int *full);
 
 /* Force pass (src/molecules.cxx:531-629): optional sort + window, M2A,
  * switched LJ + Coulomb, wall, A2M.  Sets f, n, site_f, lj, clmb, wall,

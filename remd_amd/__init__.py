@@ -82,9 +82,10 @@ def lib():
     L.remd_gpu_compute_forces.argtypes = [C.c_void_p, C.c_int, C.c_double]
     L.remd_gpu_set_debug.argtypes = [C.c_void_p, C.c_int]
     L.remd_gpu_get_site_forces.argtypes = [C.c_void_p, C.c_int, _dp]
-    L.remd_gpu_get_window.argtypes = [C.c_void_p, C.c_int, _ip, _ip]
+    L.remd_gpu_get_window.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _ip]
     L.remd_gpu_last_kernel_ms.argtypes = [C.c_void_p, _dp]
     L.remd_gpu_launch_shape.argtypes = [C.c_void_p, _ip, _ip, _ip]
+    L.remd_gpu_phase_timers.argtypes = [C.c_void_p, C.c_int, _dp]
     L.remd_gpu_autotune.argtypes = [C.c_void_p, C.c_int]
     L.remd_gpu_wall_table.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, _dp]
     L.remd_gpu_sort_window.argtypes = [_dp, C.c_int, C.c_int, C.c_double, _ip, _ip, _ip]
@@ -253,16 +254,22 @@ class REMDGPU:
         _check(lib().remd_gpu_get_site_forces(self.h, int(rep), _dp_of(out)))
         return out.reshape(self.nmol, self.nsite, 3)
 
-    def window(self, rep):
+    def window(self, rep, with_full=False):
         js = np.zeros(self.nmol, np.int32)
         je = np.zeros(self.nmol, np.int32)
-        _check(lib().remd_gpu_get_window(self.h, int(rep), _ip_of(js), _ip_of(je)))
-        return js, je
+        full = np.zeros(self.nmol, np.int32)
+        _check(lib().remd_gpu_get_window(self.h, int(rep), _ip_of(js), _ip_of(je), _ip_of(full)))
+        return (js, je, full) if with_full else (js, je)
 
     def last_kernel_ms(self):
         ms = C.c_double()
         _check(lib().remd_gpu_last_kernel_ms(self.h, C.byref(ms)))
         return ms.value
+
+    def phase_timers(self, rep=0):
+        out = np.zeros(8)
+        _check(lib().remd_gpu_phase_timers(self.h, int(rep), _dp_of(out)))
+        return out
 
     def launch_shape(self):
         a, b, c = C.c_int(), C.c_int(), C.c_int()

@@ -43,7 +43,8 @@ enum Scalar {
   S_PRS0, S_PRS1, S_PRS2, S_PRS3,
   S_VOL,
   S_STEPS,             // steps integrated so far (diagnostic)
-  NSC
+  S_DIAG0,             // 8 phase timers (REMD_PHASE_TIMERS builds only)
+  NSC = S_DIAG0 + 8
 };
 
 // kernel modes
@@ -73,6 +74,11 @@ struct JobParams {
   double lj_sig2, lj_eps4;      // site 0 LJ: sigma^2, 4 epsilon
   double unit_temp, unit_press;
   const double *fwall;          // nfwall x {f, e}
+  // FP32 pair constants, precomputed on the host so the pair pass reads
+  // them with scalar loads: rc, rl, rc^2, rl^2, 1/(rl-rc)^5, 30/(rl-rc)^5,
+  // sigma^2, 4 eps, 48 eps, q_a q_b
+  float f_rc, f_rl, f_rc2, f_rl2, f_coef, f_coef30, f_sig2, f_eps4, f_eps48;
+  float f_qq[MAXSITE][MAXSITE];
 };
 
 struct ReplicaDesc {
@@ -81,7 +87,7 @@ struct ReplicaDesc {
   int *mid;                     // nmol
   double *sc;                   // NSC doubles
   double *site_f;               // optional: nsite*nmol*3 (last force pass)
-  int *win;                     // optional: 2*nmol window (js, je)
+  int *win;                     // optional: 3*nmol window (js, je, whole-ring flag)
 };
 
 struct LaunchArgs {

@@ -239,6 +239,21 @@ static void fill_params(const remd_gpu_desc *d, JobParams &J) {
   J.lj_eps4 = 4.0 * std::sqrt(d->site_eps[0] * d->site_eps[0]);
   J.unit_temp = unit_temp();
   J.unit_press = unit_press();
+  {
+    const double rlc = d->rswitch - d->rcut;
+    const double coef = 1.0 / (rlc * rlc * rlc * rlc * rlc);
+    J.f_rc = (float)d->rcut;
+    J.f_rl = (float)d->rswitch;
+    J.f_rc2 = (float)(d->rcut * d->rcut);
+    J.f_rl2 = (float)(d->rswitch * d->rswitch);
+    J.f_coef = (float)coef;
+    J.f_coef30 = (float)(30.0 * coef);
+    J.f_sig2 = (float)J.lj_sig2;
+    J.f_eps4 = (float)J.lj_eps4;
+    J.f_eps48 = (float)(12.0 * J.lj_eps4);
+    for (int a = 0; a < MAXSITE; a++)
+      for (int b = 0; b < MAXSITE; b++) J.f_qq[a][b] = (float)(J.q[a] * J.q[b]);
+  }
 }
 
 extern "C" size_t remd_gpu_footprint(const remd_gpu_desc *d) {
@@ -298,7 +313,7 @@ static void refresh_reps(remd_gpu *g) {
     R.mid = g->d_mid + (size_t)r * g->N;
     R.sc = g->d_sc + (size_t)r * NSC;
     R.site_f = g->debug ? g->d_site + (size_t)r * g->N * g->NS * 3 : nullptr;
-    R.win = g->debug ? g->d_win + (size_t)r * g->N * 2 : nullptr;
+    R.win = g->debug ? g->d_win + (size_t)r * g->N * 3 : nullptr;
   }
 }
 
@@ -512,9 +527,9 @@ extern "C" int remd_gpu_set_debug(remd_gpu *g, int on) {
   if (!g) return fail(REMD_EINVAL, "null handle");
   if (on && !g->d_site) {
     HIPCHK(hipMalloc(&g->d_site, sizeof(double) * g->nrep * g->N * g->NS * 3));
-    HIPCHK(hipMalloc(&g->d_win, sizeof(int) * g->nrep * g->N * 2));
+    HIPCHK(hipMalloc(&g->d_win, sizeof(int) * g->nrep * g->N * 3));
     HIPCHK(hipMemsetAsync(g->d_site, 0, sizeof(double) * g->nrep * g->N * g->NS * 3, g->stream));
-    HIPCHK(hipMemsetAsync(g->d_win, 0, sizeof(int) * g->nrep * g->N * 2, g->stream));
+    HIPCHK(hipMemsetAsync(g->d_win, 0, sizeof(int) * g->nrep * g->N * 3, g->stream));
   }
   g->debug = on != 0;
   int rc = upload_reps(g);
@@ -531,14 +546,15 @@ extern "C" int remd_gpu_get_site_forces(remd_gpu *g, int rep, double *af) {
   return 0;
 }
 
-extern "C" int remd_gpu_get_window(remd_gpu *g, int rep, int *js, int *je) {
+extern "C" int remd_gpu_get_window(remd_gpu *g, int rep, int *js, int *je, int *full) {
   if (!g || !js || !je || rep < 0 || rep >= g->nrep) return fail(REMD_EINVAL, "bad argument");
   if (!g->d_win) return fail(REMD_ESTATE, "debug buffers are off");
-  std::vector<int> w(2 * g->N);
-  HIPCHK(hipMemcpy(w.data(), g->d_win + (size_t)rep * g->N * 2, sizeof(int) * 2 * g->N, hipMemcpyDeviceToHost));
+  std::vector<int> w(3 * g->N);
+  HIPCHK(hipMemcpy(w.data(), g->d_win + (size_t)rep * g->N * 3, sizeof(int) * 3 * g->N, hipMemcpyDeviceToHost));
   for (int i = 0; i < g->N; i++) {
-    js[i] = w[2 * i];
-    je[i] = w[2 * i + 1];
+    js[i] = w[3 * i];
+    je[i] = w[3 * i + 1];
+    if (full) full[i] = w[3 * i + 2];
   }
   return 0;
 }
@@ -546,6 +562,12 @@ extern "C" int remd_gpu_get_window(remd_gpu *g, int rep, int *js, int *je) {
 extern "C" int remd_gpu_last_kernel_ms(remd_gpu *g, double *ms) {
   if (!g || !ms) return fail(REMD_EINVAL, "null argument");
   *ms = g->last_ms;
+  return 0;
+}
+
+extern "C" int remd_gpu_phase_timers(remd_gpu *g, int rep, double out[8]) {
+  if (!g || !out || rep < 0 || rep >= g->nrep) return fail(REMD_EINVAL, "bad argument");
+  for (int k = 0; k < 8; k++) out[k] = g->h_sc[(size_t)rep * NSC + S_DIAG0 + k];
   return 0;
 }
 

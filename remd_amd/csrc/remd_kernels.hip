@@ -28,6 +28,23 @@ namespace remd {
 
 constexpr int WAVE = 64;
 
+// Optional phase timers (constant 100 MHz clock), accumulated by thread 0
+// into S_DIAG0.. of the replica's scalar block; compiled out by default.
+#ifdef REMD_PHASE_TIMERS
+#define PT_DECL uint64_t pt_t0 = wall_clock64();
+#define PT_MARK(k, sc)                                        \
+  do {                                                        \
+    const uint64_t pt_t1 = wall_clock64();                    \
+    if (threadIdx.x == 0) (sc)[S_DIAG0 + (k)] += (double)(pt_t1 - pt_t0); \
+    pt_t0 = pt_t1;                                            \
+  } while (0)
+#else
+#define PT_DECL
+#define PT_MARK(k, sc) \
+  do {                 \
+  } while (0)
+#endif
+
 // ------------------------------------------------------------ LDS layout
 template <int NS>
 struct Rec {
@@ -540,21 +557,35 @@ __device__ inline int count_leq(const double *z, int n, double x) {
   }
   return lo;
 }
+// The scan wraps at most twice for w < 2: a second wrap stops at once, on
+// index n - 1 (downward) or 0 (upward).
 __device__ inline void window_of(const double *z, int n, int i, double w, int &js, int &je) {
   double lb = z[i] - w;
   int k = count_less(z, n, lb);
   if (k >= 1) js = k - 1;
   else {
     lb += 1.0;
-    js = count_less(z, n, lb) - 1;
+    k = count_less(z, n, lb);
+    js = (k >= 1) ? k - 1 : n - 1;
   }
   double ub = z[i] + w;
   int m = count_leq(z, n, ub);
   if (m < n) je = m;
   else {
     ub -= 1.0;
-    je = count_leq(z, n, ub);
+    m = count_leq(z, n, ub);
+    je = (m < n) ? m : 0;
   }
+}
+
+// Contract (no reference counterpart): the window is the whole ring when
+// the scan wrapped onto i itself, or when the first key outside on one
+// side lies inside the other side's arc; the reference's js..je range then
+// misses pairs.  Same comparisons as the scan's bounds.
+__device__ inline bool window_full(const double *z, int i, double w, int js, int je) {
+  const bool a_up = (js > i) ? (z[js] <= z[i] + w) : (z[js] <= (z[i] + w) - 1.0);
+  const bool b_dn = (je < i) ? (z[je] >= z[i] - w) : (z[je] >= (z[i] - w) + 1.0);
+  return js == i || je == i || a_up || b_dn;
 }
 
 // Sort the replica's molecules by the sort-axis coordinate, permute every
@@ -599,14 +630,16 @@ __device__ void sort_and_window(const JobParams &J, double *mol, int *mid, int N
   for (int i = threadIdx.x; i < N; i += NT) {
     int js, je;
     window_of(zs, N, i, w, js, je);
+    const bool full = window_full(zs, i, w, js, je);
     if (win_dbg) {
-      win_dbg[2 * i] = js;
-      win_dbg[2 * i + 1] = je;
+      win_dbg[3 * i] = js;
+      win_dbg[3 * i + 1] = je;
+      win_dbg[3 * i + 2] = full ? 1 : 0;
     }
     if (js >= je) je += N;
-    // a scanned range that misses i means the window spans the whole box:
-    // every other molecule is a candidate
-    if (!((js <= i && i <= je) || (js <= i + N && i + N <= je))) {
+    // a whole-ring window, or a scanned range that misses i: every other
+    // molecule is a candidate
+    if (full || !((js <= i && i <= je) || (js <= i + N && i + N <= je))) {
       js = i + 1;
       je = i + N - 1;
     }
@@ -747,31 +780,47 @@ __device__ void m2a(const JobParams &J, const double *mol, int N, double L0, dou
 // into the molecule block and the energy/virial totals into c.
 template <int NT, int NS, typename PT>
 __device__ void force_pass(const JobParams &J, double *mol, int N, Sc &c, const Lds &S,
-                           double *site_dbg) {
+                           double *site_dbg, double *pt_sc) {
+  PT_DECL
   constexpr int QB = (NS == 3) ? 0 : 1;
   constexpr int ST = Rec<NS>::STRIDE;
   PT *rec = (PT *)S.region0;
   m2a<NT, NS, PT>(J, mol, N, c.L[0], c.L[1], c.L[2], rec);
   __syncthreads();
+  PT_MARK(4, pt_sc);
 
   PairK<NS, PT> k;
-  k.rc = (PT)J.rcut;
-  k.rl = (PT)J.rswitch;
-  k.rc2 = (PT)(J.rcut * J.rcut);
-  k.rl2 = (PT)(J.rswitch * J.rswitch);
-  {
+  if (sizeof(PT) == 4) {
+    k.rc = J.f_rc;
+    k.rl = J.f_rl;
+    k.rc2 = J.f_rc2;
+    k.rl2 = J.f_rl2;
+    k.coef = J.f_coef;
+    k.coef30 = J.f_coef30;
+    k.sig2 = J.f_sig2;
+    k.eps4 = J.f_eps4;
+    k.eps48 = J.f_eps48;
+#pragma unroll
+    for (int a = 0; a < NS; a++)
+#pragma unroll
+      for (int b = 0; b < NS; b++) k.qq[a][b] = J.f_qq[a][b];
+  } else {
+    k.rc = (PT)J.rcut;
+    k.rl = (PT)J.rswitch;
+    k.rc2 = (PT)(J.rcut * J.rcut);
+    k.rl2 = (PT)(J.rswitch * J.rswitch);
     const double rlc = J.rswitch - J.rcut;
     const double coef = 1.0 / (rlc * rlc * rlc * rlc * rlc);
     k.coef = (PT)coef;
     k.coef30 = (PT)(30.0 * coef);
+    k.sig2 = (PT)J.lj_sig2;
+    k.eps4 = (PT)J.lj_eps4;
+    k.eps48 = (PT)(12.0 * J.lj_eps4);
+#pragma unroll
+    for (int a = 0; a < NS; a++)
+#pragma unroll
+      for (int b = 0; b < NS; b++) k.qq[a][b] = (PT)(J.q[a] * J.q[b]);
   }
-  k.sig2 = (PT)J.lj_sig2;
-  k.eps4 = (PT)J.lj_eps4;
-  k.eps48 = (PT)(12.0 * J.lj_eps4);
-#pragma unroll
-  for (int a = 0; a < NS; a++)
-#pragma unroll
-    for (int b = 0; b < NS; b++) k.qq[a][b] = (PT)(J.q[a] * J.q[b]);
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     k.L[d] = (PT)c.L[d];
@@ -904,8 +953,10 @@ __device__ void force_pass(const JobParams &J, double *mol, int N, Sc &c, const 
       vir[2] += acc.vir[2];
     }
   }
+  PT_MARK(5, pt_sc);
   double tot[6] = {e_lj, e_cl, e_wall, vir[0], vir[1], vir[2]};
   block_sum<NT, 6>(tot, S.red);
+  PT_MARK(6, pt_sc);
   c.lj = 0.5 * tot[0];
   c.clmb = 0.5 * tot[1];
   c.wall = tot[2];
@@ -919,10 +970,14 @@ __device__ void force_pass(const JobParams &J, double *mol, int N, Sc &c, const 
 template <int NT, int NS, typename PT>
 __device__ void md_step(const JobParams &J, double *mol, int *mid, int N, Sc &c, const Lds &S,
                         bool do_sort, const ReplicaDesc &R) {
+  PT_DECL
   // D6, D5 scalars, D4 (src/integrator.cxx:651-658)
   D6(J, c);
-
-  Sc c_old = c;
+  Sc c_old;  // D5 on the molecules uses the box and s from before D4
+  c_old.s = c.s;
+  c_old.L[0] = c.L[0];
+  c_old.L[1] = c.L[1];
+  c_old.L[2] = c.L[2];
   D5_scalars(J, c);
   D4(J, c);
   // per molecule: D5 (old box), D3 D2 D1 D2 D3 (new box)
@@ -989,9 +1044,12 @@ __device__ void md_step(const JobParams &J, double *mol, int *mid, int N, Sc &c,
   }
   if (T1) c.Ps += (0.25 * (sums[0] + sums[1] + sums[2]) + 2.0 * sums[3] - (1.0 + log(c.s)) * c.gkT + c.H0) * J.dt;
   D4(J, c);
+  PT_MARK(0, R.sc);
   // force
   if (do_sort) sort_and_window<NT>(J, mol, mid, N, J.sort_axis == 0 ? c.L[0] : c.L[2], J.margin, S, R.win);
-  force_pass<NT, NS, PT>(J, mol, N, c, S, R.site_f);  // ends with block_sum (synced)
+  PT_MARK(1, R.sc);
+  force_pass<NT, NS, PT>(J, mol, N, c, S, R.site_f, R.sc);  // ends with block_sum (synced)
+  PT_MARK(2, R.sc);
   // D5, D6
   for (int i = threadIdx.x; i < N; i += NT) {
     double v[3] = {mol[F_VX * N + i], mol[F_VY * N + i], mol[F_VZ * N + i]};
@@ -1006,6 +1064,7 @@ __device__ void md_step(const JobParams &J, double *mol, int *mid, int N, Sc &c,
   D5_scalars(J, c);
   D6(J, c);
   __syncthreads();
+  PT_MARK(3, R.sc);
 }
 
 // --------------------------------------------------------- replica driver
@@ -1022,7 +1081,7 @@ __device__ void run_replica(const LaunchArgs &a, const ReplicaDesc &R, const Job
     // InitializeProperty (src/molecules.h:260-270): sort with margin 0,
     // force, gkT, CalcProperties, H0 = tot.
     sort_and_window<NT>(J, mol, mid, N, J.sort_axis == 0 ? c.L[0] : c.L[2], 0.0, S, R.win);
-    force_pass<NT, NS, PT>(J, mol, N, c, S, R.site_f);
+    force_pass<NT, NS, PT>(J, mol, N, c, S, R.site_f, R.sc);
     c.gkT = 6.0 * (double)N * c.T;
     const double tot = total_energy<NT>(J, mol, N, c, S.red);
     c.H0 = tot;
@@ -1035,10 +1094,11 @@ __device__ void run_replica(const LaunchArgs &a, const ReplicaDesc &R, const Job
     else {
       // reuse the stored window
       for (int i = threadIdx.x; i < N; i += NT) {
-        int js = R.win ? R.win[2 * i] : 0;
-        int je = R.win ? R.win[2 * i + 1] : N - 1;
+        int js = R.win ? R.win[3 * i] : 0;
+        int je = R.win ? R.win[3 * i + 1] : N - 1;
+        const bool full = R.win ? R.win[3 * i + 2] != 0 : true;
         if (js >= je) je += N;
-        if (!((js <= i && i <= je) || (js <= i + N && i + N <= je))) {
+        if (full || !((js <= i && i <= je) || (js <= i + N && i + N <= je))) {
           js = i + 1;
           je = i + N - 1;
         }
@@ -1047,7 +1107,7 @@ __device__ void run_replica(const LaunchArgs &a, const ReplicaDesc &R, const Job
       }
       __syncthreads();
     }
-    force_pass<NT, NS, PT>(J, mol, N, c, S, R.site_f);
+    force_pass<NT, NS, PT>(J, mol, N, c, S, R.site_f, R.sc);
     calc_properties<NT>(J, mol, N, c, R.sc, S.red);
     if (threadIdx.x == 0) store_sc(c, R.sc);
     return;

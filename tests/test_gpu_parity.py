@@ -88,10 +88,15 @@ def test_step0_forces_reference_fixture(golden, prec, tol_f, tol_e):
 
 
 @pytest.mark.parametrize("model,conf", [("TIP3P", 1), ("TIP4P", 1), ("TIP5P", 1), ("TIP4P", 0), ("TIP5P", 2),
-                                        ("TIP3P", 2)])
+                                        ("TIP3P", 2), ("TIP4P", 3)])
 @pytest.mark.parametrize("prec", [0, 1])
 def test_step0_forces_vs_oracle(model, conf, prec):
-    if conf == 0:
+    """conf 3: 64 molecules on two lattice planes of a bulk box, so every
+    window is the whole ring (the whole-ring window rule)."""
+    if conf == 3:
+        sysm = inp.config_bulk(seed=11, nmol=64, ntemp=2, model=model)
+        conf = 0
+    elif conf == 0:
         sysm = inp.config_bulk(seed=11, ntemp=2, model=model)
     elif conf == 1:
         sysm = inp.config_tube(seed=12, nmol=120, ntemp=2, npress=1, model=model)
@@ -107,8 +112,9 @@ def test_step0_forces_vs_oracle(model, conf, prec):
         so = o.state()
         sg = g.state(rep)
         assert np.array_equal(sg["id"], so["id"])
-        js, je = g.window(rep)
+        js, je, full = g.window(rep, with_full=True)
         assert np.array_equal(js, so["jstart"]) and np.array_equal(je, so["jend"])
+        assert np.array_equal(full, so["wfull"])
         ef = rel_rms(sg["f"], so["f"])
         en = rel_rms(sg["n"], so["n"])
         esf = rel_rms(g.site_forces(rep), so["site_f"])
