@@ -447,3 +447,138 @@ def test_pair_lists_are_disjoint_neighbours():
                 ta, pa = a % nt, a // nt
                 tb, pb = b % nt, b // nt
                 assert (abs(ta - tb) == 1 and pa == pb) or (abs(pa - pb) == 1 and ta == tb)
+
+
+# --------------------------------------------- walls on a lone molecule
+@pytest.mark.parametrize("conf", [1, 2])
+def test_wall_force_is_minus_wall_energy_gradient(conf):
+    """One molecule, no pair partner: the whole energy is the wall's.  The
+    table lookup applies f(r) on the oxygen site; its COM force equals
+    -dE_wall/dx of the interpolated energy up to the table's own
+    first-order consistency (relative 2e-3)."""
+    if conf == 1:
+        s = inp.System("TIP4P", 1, 0, 0, 1, 2.0, 9.0, 8.0, 1, 1, 1, np.array([2.0]), np.array([0.0]),
+                       wall_length=7.0, nfwall=20000, sgm_wall=SGM, eps_wall=EPS, rho_wall=RHO)
+        L = (44.752, 44.752, 50.0)
+        r0 = np.array([[0.5 + 3.9 / L[0], 0.5 - 1.2 / L[1], 0.3]])
+        axes = (0, 1)
+    else:
+        s = inp.System("TIP5P", 1, 0, 0, 2, 2.0, 9.0, 8.0, 1, 1, 1, np.array([2.0]), np.array([0.0]),
+                       wall_length=10.0, nfwall=4000, sgm_wall=SGM, eps_wall=EPS, rho_wall=RHO)
+        L = (40.0, 40.0, 50.0)
+        r0 = np.array([[0.3, 0.6, 0.5 + 2.2 / L[2]]])
+        axes = (2,)
+    q = np.array([[0.9, 0.1, -0.3, 0.2]])
+    q /= np.linalg.norm(q)
+    s.states = [inp.ReplicaState(r=r0, v=np.zeros((1, 3)), q=q, p=np.zeros((1, 4)), L=L,
+                                 ids=np.zeros(1, np.int32))]
+    o = oracle_for(s)
+    o.calc_force(True, 0.0)
+    F = o.state()["f"][0]
+    assert o.scalars()["wall"] != 0.0
+    for d in axes:
+        h = 2e-4
+        e = []
+        for sgn in (+1, -1):
+            r = r0.copy()
+            r[0, d] += sgn * h / L[d]
+            o.set_state(r, np.zeros((1, 3)), q, np.zeros((1, 4)))
+            o.calc_force(True, 0.0)
+            e.append(o.scalars()["wall"])
+        g = (e[0] - e[1]) / (2 * h)
+        assert -g == pytest.approx(F[d], rel=2e-3)
+
+
+# ------------------------------------------------- exchange rescale laws
+def test_change_temperature_scales_kinetic_energy():
+    """ChangeTemperature (src/molecules.h:333-353): after a swap T1 -> T2
+    both kinetic energies scale by exactly T2/T1 and s returns to 1."""
+    d = np.load(__import__("os").path.join(__import__("os").path.dirname(__file__), "golden", "ref_tube_chk.npz"))
+    o = oracle_for(fixture_system(d))
+    o.initialize_property()
+    o.execute_steps(20)
+    o.calc_properties()
+    a = o.scalars()
+    T1 = a["T"]
+    T2 = 1.37 * T1
+    o.change_temperature(T2)
+    o.calc_properties()
+    b = o.scalars()
+    assert b["s"] == 1.0 and b["T"] == T2
+    assert b["tra"] == pytest.approx(a["tra"] * T2 / T1, rel=1e-12)
+    # p . P_k q cancels against the checkpoint's p . q component: 1e-8
+    assert b["rot"] == pytest.approx(a["rot"] * T2 / T1, rel=1e-8)
+
+
+@pytest.mark.parametrize("conf", [0, 1, 2])
+def test_kill_momentum_removes_the_free_momenta(conf):
+    """KillMomentum (src/molecules.cxx:168-210): the momenta along the
+    periodic axes vanish; in the tube also the angular momentum about the
+    tube axis."""
+    if conf == 0:
+        s = inp.config_bulk(seed=3, nmol=64, ntemp=1, npress=1)
+        free = (0, 1, 2)
+    elif conf == 1:
+        s = inp.config_tube(seed=3, nmol=60, ntemp=1, npress=1)
+        free = (2,)
+    else:
+        s = inp.config_slit(seed=3, nmol=80, ntemp=1, npress=1, Lxy=30.0)
+        free = (0, 1)
+    o = oracle_for(s)
+    st = o.state()
+    v = st["v"] + 0.3  # add a drift on every axis
+    o.set_state(st["r"], v, st["q"], st["p"])
+    st0r = st["r"].copy()
+    v0_after_linear = v.copy()
+    v0_after_linear[:, 2] -= v0_after_linear[:, 2].mean()
+    o.kill_momentum()
+    st = o.state()
+    L = np.array(o.scalars()["L"])
+    vel = st["v"] / L  # s = 1
+    scale = np.abs(vel).max()
+    for d in free:
+        assert abs(vel[:, d].sum()) <= 1e-12 * scale * len(vel)
+    if conf == 1:
+        # the tube's angular momentum about the axis: each molecule further
+        # than 1 A from the axis takes -<l>, those inside keep theirs
+        # (src/molecules.cxx:189-210), so the residual is <l> x n_inside
+        x = (st["r"] - 0.5) * L
+        lz = np.sum(x[:, 0] * vel[:, 1] - x[:, 1] * vel[:, 0])
+        x0 = (st0r - 0.5) * L
+        v0 = v0_after_linear / L
+        l_mean = np.mean(x0[:, 0] * v0[:, 1] - x0[:, 1] * v0[:, 0])
+        n_in = int(np.sum(x0[:, 0] ** 2 + x0[:, 1] ** 2 <= 1.0))
+        assert abs(lz - l_mean * n_in) <= 1e-10 * scale * L[0] * len(vel)
+
+
+# ------------------------------------------- barostats: conserved H
+@pytest.mark.parametrize("conf", [0, 2])
+def test_isotropic_and_area_barostats_conserve_hamiltonian(conf):
+    """NP thermostat with the 3D isotropic (bulk, pconstant 1) or 2D area
+    (slit, pconstant 3) Andersen barostat: the extended Hamiltonian's
+    deviation shrinks ~4x when dt halves (second-order splitting), which a
+    wrong barostat kinetic term or pressure force would break."""
+    if conf == 0:
+        base = inp.config_bulk(seed=21, nmol=125, L=16.0, ntemp=1, npress=1)
+        pmode = 1
+    else:
+        base = inp.config_slit(seed=22, nmol=100, Lxy=24.0, ntemp=1, npress=1, model="TIP4P")
+        pmode = 3
+    from helpers import relax_oracle
+    base = relax_oracle(base, steps=200, dt_fs=0.5)
+    devs = []
+    for dt, n in ((1.0, 60), (0.5, 120)):
+        s = __import__("copy").copy(base)
+        s.tconstant, s.pconstant, s.dt_fs = 1, pmode, dt
+        s.P = np.array([50e6 / inp.UNIT_PRESS])
+        o = oracle_for(s)
+        o.initialize_property()
+        dev = 0.0
+        L0 = list(o.scalars()["L"])
+        for _ in range(n):
+            o.execute_steps(1)
+            o.calc_properties()
+            dev = max(dev, abs(o.scalars()["ham"]))
+        assert list(o.scalars()["L"]) != L0  # the box moved
+        devs.append(dev)
+    assert 2.8 < devs[0] / devs[1] < 5.5, devs
