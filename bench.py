@@ -90,7 +90,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run 3 device-to-device copies of this many MiB (PMC counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe_mib > 0:
+        import torch
+        n = args.copy_probe_mib << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n, "copies": 3}))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: replicas shard across GPUs as whole jobs/ladders with no "
               "per-step communication, so only --gpus 1 is run here.", file=sys.stderr)
@@ -120,6 +132,9 @@ def main():
 
     for _ in range(args.warmup):
         one_interval()
+    path, parts = job.launch_path()
+    kname = ("resident_kernel<256,4,float>" if path == ra.PATH_RESIDENT else
+             f"interval_kernel<{shape['block_threads']},4,float>")
 
     # algorithmic work: ordered in-cutoff pairs at the interval-start snapshot
     flops_per_launch = workloads.algorithmic_flops(job, interval)
@@ -181,8 +196,9 @@ def main():
             "workload": "nanotube TIP4P, 144 mol/replica, R 7 A, Lz 50 A, 16T x 4P = 64 replicas, "
                         "rcut 9 A, dt 2 fs, 100 steps/interval, NP thermostat + 1D barostat",
             "replicas": nrep, "nmol": system.nmol, "steps_per_interval": interval,
-            "launch": {"block_threads": shape["block_threads"], "chunks": shape["chunks"],
-                       "lds_bytes": shape["lds_bytes"]},
+            "launch": {"path": "resident" if path == ra.PATH_RESIDENT else "global",
+                       "workgroups_per_replica": parts, "block_threads": shape["block_threads"],
+                       "chunks": shape["chunks"], "lds_bytes": shape["lds_bytes"]},
         },
         "plain_value": None if plain is None else round(plain, 1),
         "batched_over_plain": None if plain is None else round(value / plain, 2),
@@ -192,7 +208,7 @@ def main():
                                 "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
         "roofline": {
             "bound": "valu",
-            "kernel": "interval_kernel<256,4,float>",
+            "kernel": kname,
             "achieved": round(achieved, 4),
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",

@@ -63,7 +63,12 @@ typedef struct {
   int precision;         /* REMD_PREC_MIXED: FP32 pair math, FP64 accumulation */
   int block_threads;     /* 0: default (256); 128, 256 or 512 */
   int chunks;            /* j-chunks per molecule in the pair pass; 0: heuristic */
+  int path;              /* REMD_PATH_AUTO, _RESIDENT (state in LDS, replicas may span
+                            several workgroups) or _GLOBAL (state in HBM, one workgroup
+                            per replica); AUTO takes RESIDENT when the replica fits */
 } remd_gpu_desc;
+
+enum { REMD_PATH_AUTO = 0, REMD_PATH_RESIDENT = 1, REMD_PATH_GLOBAL = 2 };
 
 /* Per-replica extended variables (src/variables.h:137-157) and box. */
 typedef struct {
@@ -132,6 +137,8 @@ int remd_gpu_last_kernel_ms(remd_gpu *g, double *ms);
 int remd_gpu_phase_timers(remd_gpu *g, int rep, double out[8]);
 /* Chosen launch shape. */
 int remd_gpu_launch_shape(remd_gpu *g, int *block_threads, int *chunks, int *lds_bytes);
+/* Path of the job and the workgroups per replica its last launch used. */
+int remd_gpu_launch_path(remd_gpu *g, int *path, int *parts);
 /* Measure each (block_threads, chunks) candidate with timed force passes and
  * keep the fastest (project autotuner). */
 int remd_gpu_autotune(remd_gpu *g, int reps_per_candidate);

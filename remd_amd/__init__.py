@@ -41,6 +41,7 @@ class Desc(C.Structure):
         ("site_q", C.c_double * 5),
         ("mass", C.c_double), ("inertia", C.c_double * 3),
         ("precision", C.c_int), ("block_threads", C.c_int), ("chunks", C.c_int),
+        ("path", C.c_int),
     ]
 
 
@@ -85,6 +86,7 @@ def lib():
     L.remd_gpu_get_window.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _ip]
     L.remd_gpu_last_kernel_ms.argtypes = [C.c_void_p, _dp]
     L.remd_gpu_launch_shape.argtypes = [C.c_void_p, _ip, _ip, _ip]
+    L.remd_gpu_launch_path.argtypes = [C.c_void_p, _ip, _ip]
     L.remd_gpu_phase_timers.argtypes = [C.c_void_p, C.c_int, _dp]
     L.remd_gpu_autotune.argtypes = [C.c_void_p, C.c_int]
     L.remd_gpu_wall_table.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, _dp]
@@ -125,7 +127,10 @@ def _ip_of(a):
     return a.ctypes.data_as(_ip)
 
 
-def desc_for(system: "inputs.System", precision=0, block_threads=0, chunks=0) -> Desc:
+PATH_AUTO, PATH_RESIDENT, PATH_GLOBAL = 0, 1, 2
+
+
+def desc_for(system: "inputs.System", precision=0, block_threads=0, chunks=0, path=PATH_AUTO) -> Desc:
     """remd_gpu_desc for an inputs.System (reduced units)."""
     mt = inputs.molecule(system.model)
     d = Desc()
@@ -151,19 +156,20 @@ def desc_for(system: "inputs.System", precision=0, block_threads=0, chunks=0) ->
     for k in range(3):
         d.inertia[k] = mt.inertia[k]
     d.precision, d.block_threads, d.chunks = precision, block_threads, chunks
+    d.path = path
     return d
 
 
 class REMDGPU:
     """Mirror of REMDGPU (src/remdgpu.h:92-238) for one job."""
 
-    def __init__(self, system: "inputs.System", precision=0, block_threads=0, chunks=0):
+    def __init__(self, system: "inputs.System", precision=0, block_threads=0, chunks=0, path=PATH_AUTO):
         L = lib()
         self.system = system
         self.nrep = system.nreplica
         self.nmol = system.nmol
         self.nsite = inputs.NSITE[system.model]
-        self.desc = desc_for(system, precision, block_threads, chunks)
+        self.desc = desc_for(system, precision, block_threads, chunks, path)
         R, N = self.nrep, self.nmol
         r = np.ascontiguousarray(np.stack([s.r for s in system.states]).reshape(R * N, 3))
         v = np.ascontiguousarray(np.stack([s.v for s in system.states]).reshape(R * N, 3))
@@ -275,6 +281,13 @@ class REMDGPU:
         a, b, c = C.c_int(), C.c_int(), C.c_int()
         _check(lib().remd_gpu_launch_shape(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return dict(block_threads=a.value, chunks=b.value, lds_bytes=c.value)
+
+    def launch_path(self):
+        """(path, parts): PATH_RESIDENT or PATH_GLOBAL, and the workgroups per
+        replica of the last launch."""
+        p, g = C.c_int(), C.c_int()
+        _check(lib().remd_gpu_launch_path(self.h, C.byref(p), C.byref(g)))
+        return p.value, g.value
 
     def autotune(self, reps=3):
         _check(lib().remd_gpu_autotune(self.h, int(reps)))

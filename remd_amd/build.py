@@ -53,8 +53,9 @@ def build_driver():
     if not os.path.exists(src):
         return None
     tmp = DRIVER + ".tmp"
-    _run([HIPCC, "-O2", "-std=c++17", "-o", tmp, src, f"-L{PKG}", "-lremd_gpu",
-          f"-Wl,-rpath,$ORIGIN", "-Wno-unused-result"])
+    # host-only C++ over the C ABI
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-o", tmp, src, f"-I{os.path.join(ROOT, 'include')}",
+          f"-L{PKG}", "-lremd_gpu", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, DRIVER)
     return DRIVER
 

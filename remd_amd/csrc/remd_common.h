@@ -61,7 +61,8 @@ struct JobParams {
   int per[3];                   // periodic axes
   int interval, sort_interval;
   int nfwall;
-  int chunks;                   // j-chunks per i-molecule in the pair pass (power of two)
+  int chunks;                   // j-chunks per i-molecule in the pair pass (power of two on the
+                                // global-state path, any 1..64 on the resident path)
   int precision;                // 0 mixed (FP32 pair, FP64 acc), 1 FP64
   int pad0;
   double dt, dthalf;
@@ -88,6 +89,7 @@ struct ReplicaDesc {
   double *sc;                   // NSC doubles
   double *site_f;               // optional: nsite*nmol*3 (last force pass)
   int *win;                     // optional: 3*nmol window (js, je, whole-ring flag)
+  double *xbuf;                 // resident path with parts > 1: 2 x nmol x 16 exchange rows
 };
 
 struct LaunchArgs {
@@ -96,6 +98,9 @@ struct LaunchArgs {
   int mode;
   int do_sort;                  // MODE_FORCE only
   double margin;                // MODE_FORCE only
+  int parts;                    // resident path: workgroups per replica
+  unsigned *sync;               // resident path, parts > 1: nrep arrival counters, zeroed
+  unsigned *abort;              // resident path, parts > 1: abort word, zeroed
 };
 
 }  // namespace remd

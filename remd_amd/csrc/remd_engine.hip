@@ -19,6 +19,9 @@
 namespace remd {
 hipError_t launch_interval(const LaunchArgs &a, int nt, int nsite, int prec, int lds, hipStream_t st);
 hipError_t set_kernel_lds_limit(int bytes);
+hipError_t launch_resident(const LaunchArgs &a, int nsite, int prec, int lds, hipStream_t st);
+hipError_t resident_occupancy(int nsite, int prec, int lds, int *per_cu);
+int resident_lds(int nmol, int nsite, int prec);
 hipError_t launch_wall_tube(int nfwall, double R, double sgm06, double coefE, double *fw, hipStream_t st);
 hipError_t launch_wall_slit(int nfwall, double H, double sgm, double eps, double rho, double *fw,
                             hipStream_t st);
@@ -92,11 +95,21 @@ static int device_lds_limit() {
 }
 
 // -------------------------------------------------------------- job
+// Largest number of workgroups one replica is split over (resident path).
+constexpr int GMAX = 8;
+constexpr int XW = 16;  // doubles per molecule row of the force exchange
+
 struct remd_gpu {
   remd_gpu_desc desc;
   JobParams prm;
   int nrep = 0, N = 0, NS = 0;
   int nt = 256, lds = 0;
+  bool resident = false;     // state in LDS for the launch (resident_kernel)
+  int capacity = 0;          // resident workgroups the device holds at once
+  int last_parts = 1;
+  unsigned *d_sync = nullptr;  // nrep arrival counters + abort word
+  double *d_xbuf = nullptr;    // nrep x 2 x N x XW exchange rows
+  unsigned *h_abort = nullptr; // pinned copy of the abort word
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0.0;
@@ -121,10 +134,12 @@ struct remd_gpu {
   ExchangeJob *d_xjob = nullptr;
 };
 
-// Kernel class = (sites, pair precision); one launch per class, each on its
-// own stream so classes run concurrently.
-constexpr int NCLASS = 6;
-static int kclass(int nsite, int prec) { return (nsite - 3) * 2 + prec; }
+// Kernel class = (path, sites, pair precision); one launch per class, each
+// on its own stream.  Resident classes run concurrently and share the
+// co-resident workgroup slots; global-state classes start after them, so a
+// long global-state launch never holds the CUs a waiting part needs.
+constexpr int NCLASS = 12;
+static int kclass(bool resident, int nsite, int prec) { return (resident ? 6 : 0) + (nsite - 3) * 2 + prec; }
 
 struct remd_batch {
   std::vector<remd_gpu *> jobs;
@@ -134,10 +149,13 @@ struct remd_batch {
   bool dirty = true;
   int nt = 256;
   int cls_off[NCLASS] = {0}, cls_n[NCLASS] = {0}, cls_lds[NCLASS] = {0};
+  int res_capacity = 0, res_minN = 0, res_reps = 0, parts = 1;
+  unsigned *d_sync = nullptr;  // ntab arrival counters + one abort word
+  unsigned *h_abort = nullptr;
   hipStream_t stream = nullptr;
   hipStream_t cstream[NCLASS] = {nullptr};
   hipEvent_t cdone[NCLASS] = {nullptr};
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, evres = nullptr;
   double last_ms = 0.0;
 };
 
@@ -181,8 +199,8 @@ static int validate(const remd_gpu_desc *d, const remd_gpu_scalars *sc) {
     return fail(REMD_EINVAL, "4/5-site models must have an uncharged site 0");
   if (d->block_threads != 0 && d->block_threads != 128 && d->block_threads != 256 && d->block_threads != 512)
     return fail(REMD_EINVAL, "block_threads must be 0, 128, 256 or 512");
-  if (d->chunks != 0 && (d->chunks < 1 || d->chunks > 64 || (d->chunks & (d->chunks - 1))))
-    return fail(REMD_EINVAL, "chunks must be 0 or a power of two <= 64");
+  if (d->chunks < 0 || d->chunks > 64) return fail(REMD_EINVAL, "chunks must be in [0, 64]");
+  if (d->path < 0 || d->path > 2) return fail(REMD_EINVAL, "path must be 0 (auto), 1 (resident) or 2 (global)");
   if (d->precision != 0 && d->precision != 1) return fail(REMD_EINVAL, "precision must be 0 or 1");
   const int axis = (Cm == 2) ? 0 : 2;
   for (int r = 0; r < d->nreplica; r++) {
@@ -282,6 +300,9 @@ static void free_job(remd_gpu *g) {
   hipFree(g->d_scptr);
   hipFree(g->d_acc);
   hipFree(g->d_xjob);
+  hipFree(g->d_sync);
+  hipFree(g->d_xbuf);
+  if (g->h_abort) hipHostFree(g->h_abort);
   if (g->h_sc) hipHostFree(g->h_sc);
   if (g->ev0) hipEventDestroy(g->ev0);
   if (g->ev1) hipEventDestroy(g->ev1);
@@ -314,6 +335,7 @@ static void refresh_reps(remd_gpu *g) {
     R.sc = g->d_sc + (size_t)r * NSC;
     R.site_f = g->debug ? g->d_site + (size_t)r * g->N * g->NS * 3 : nullptr;
     R.win = g->debug ? g->d_win + (size_t)r * g->N * 3 : nullptr;
+    R.xbuf = g->d_xbuf ? g->d_xbuf + (size_t)r * 2 * g->N * XW : nullptr;
   }
 }
 
@@ -324,6 +346,18 @@ static int upload_reps(remd_gpu *g) {
   return 0;
 }
 
+// Workgroups per replica for a resident launch of nrep replicas that has
+// `capacity` co-resident workgroup slots to itself: every part of every
+// replica must be resident at once (the parts wait for each other), and a
+// part keeps at least 8 molecules.
+static int pick_parts(int capacity, int nrep, int N) {
+  if (nrep <= 0) return 1;
+  int p = capacity / nrep;
+  p = std::min(p, GMAX);
+  p = std::min(p, std::max(1, N / 8));
+  return p >= 2 ? p : 1;
+}
+
 static int launch(remd_gpu *g, int mode, int do_sort, double margin, const ReplicaDesc *reps, int nrep,
                   bool timed) {
   LaunchArgs a;
@@ -332,16 +366,38 @@ static int launch(remd_gpu *g, int mode, int do_sort, double margin, const Repli
   a.mode = mode;
   a.do_sort = do_sort;
   a.margin = margin;
+  a.parts = 1;
+  a.sync = g->d_sync;
+  a.abort = g->d_sync ? g->d_sync + g->nrep : nullptr;
+  if (g->resident) {
+    a.parts = pick_parts(g->capacity, nrep, g->N);
+    if (a.parts > 1) HIPCHK(hipMemsetAsync(g->d_sync, 0, sizeof(unsigned) * (nrep + 1), g->stream));
+  }
+  g->last_parts = a.parts;
   if (timed) HIPCHK(hipEventRecord(g->ev0, g->stream));
-  HIPCHK(launch_interval(a, g->nt, g->NS, g->prm.precision, g->lds, g->stream));
+  if (g->resident) HIPCHK(launch_resident(a, g->NS, g->prm.precision, g->lds, g->stream));
+  else HIPCHK(launch_interval(a, g->nt, g->NS, g->prm.precision, g->lds, g->stream));
   if (timed) HIPCHK(hipEventRecord(g->ev1, g->stream));
+  if (a.parts > 1)
+    HIPCHK(hipMemcpyAsync(g->h_abort, a.abort, sizeof(unsigned), hipMemcpyDeviceToHost, g->stream));
+  return 0;
+}
+
+// Stream sync + the launch's abort word: a resident launch whose parts could
+// not all be resident gives up (no state is written back) instead of waiting.
+static int finish(remd_gpu *g) {
+  HIPCHK(hipStreamSynchronize(g->stream));
+  if (g->last_parts > 1 && *g->h_abort != 0u) {
+    *g->h_abort = 0u;
+    return fail(REMD_EHIP, "resident launch aborted: the parts of a replica were not co-resident "
+                           "(another kernel held the device)");
+  }
   return 0;
 }
 
 static int sync_scalars(remd_gpu *g) {
   HIPCHK(hipMemcpyAsync(g->h_sc, g->d_sc, sizeof(double) * NSC * g->nrep, hipMemcpyDeviceToHost, g->stream));
-  HIPCHK(hipStreamSynchronize(g->stream));
-  return 0;
+  return finish(g);
 }
 
 extern "C" int remd_gpu_create(remd_gpu **out, const remd_gpu_desc *d, const remd_gpu_scalars *sc,
@@ -359,18 +415,47 @@ extern "C" int remd_gpu_create(remd_gpu **out, const remd_gpu_desc *d, const rem
   g->NS = d->nsite;
   fill_params(d, g->prm);
   g->nt = d->block_threads ? d->block_threads : 256;
-  // window fraction along the sort axis ~ 2 (rcut + margin) / L
-  {
-    const int axis = g->prm.sort_axis;
-    double wf = std::min(1.0, 2.0 * (d->rcut + d->margin) / sc[0].L[axis]);
-    g->prm.chunks = d->chunks ? d->chunks : pick_chunks(g->N, g->nt, wf);
-  }
-  g->lds = lds_bytes_host(g->N, g->NS, g->prm.precision, g->nt);
   const int lim = device_lds_limit();
-  if (g->lds > lim) {
+  // window fraction along the sort axis ~ 2 (rcut + margin) / L
+  const double wf = std::min(1.0, 2.0 * (d->rcut + d->margin) / sc[0].L[g->prm.sort_axis]);
+  const int rlds = resident_lds(g->N, g->NS, g->prm.precision);
+  const bool fits = g->nt == 256 && rlds <= lim;
+  if (d->path == REMD_PATH_RESIDENT && !fits) {
     free_job(g);
-    return fail(REMD_EINVAL, "replica too large for workgroup LDS (" + std::to_string(g->lds) + " > " +
-                                 std::to_string(lim) + " bytes)");
+    return fail(REMD_EINVAL, "replica does not fit the resident path (" + std::to_string(rlds) + " > " +
+                                 std::to_string(lim) + " bytes of LDS, or block_threads != 256)");
+  }
+  g->resident = d->path == REMD_PATH_RESIDENT || (d->path == REMD_PATH_AUTO && fits);
+  if (g->resident) {
+    g->lds = rlds;
+    int per_cu = 0, ncu = 0, dev = 0;
+    hipGetDevice(&dev);
+    if (resident_occupancy(g->NS, g->prm.precision, rlds, &per_cu) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || per_cu < 1 ||
+        ncu < 1) {
+      free_job(g);
+      return fail(REMD_EHIP, "occupancy query for the resident kernel failed");
+    }
+    g->capacity = per_cu * ncu;
+    // chunks: the (molecule, chunk) items of one part fill its 256 lanes at
+    // the part count this job gets alone; fixed for the job's lifetime, so
+    // results do not depend on the part count of a later launch
+    const int gn = pick_parts(g->capacity, g->nrep, g->N);
+    const int M = (g->N + gn - 1) / gn;
+    const int wcount = std::max(1, (int)std::ceil(wf * g->N));
+    g->prm.chunks = d->chunks ? d->chunks : std::max(1, std::min(std::min(64, 256 / M), wcount));
+  } else {
+    if (d->chunks && (d->chunks & (d->chunks - 1))) {
+      free_job(g);
+      return fail(REMD_EINVAL, "chunks must be a power of two on the global-state path");
+    }
+    g->prm.chunks = d->chunks ? d->chunks : pick_chunks(g->N, g->nt, wf);
+    g->lds = lds_bytes_host(g->N, g->NS, g->prm.precision, g->nt);
+    if (g->lds > lim) {
+      free_job(g);
+      return fail(REMD_EINVAL, "replica too large for workgroup LDS (" + std::to_string(g->lds) + " > " +
+                                   std::to_string(lim) + " bytes)");
+    }
   }
 #define CK(...)                        \
   do {                                 \
@@ -392,6 +477,13 @@ extern "C" int remd_gpu_create(remd_gpu **out, const remd_gpu_desc *d, const rem
     HIPCHK(hipMalloc(&g->d_sc, sizeof(double) * NSC * R));
     HIPCHK(hipMalloc(&g->d_reps, sizeof(ReplicaDesc) * R));
     HIPCHK(hipHostMalloc(&g->h_sc, sizeof(double) * NSC * R, hipHostMallocDefault));
+    if (g->resident) {
+      HIPCHK(hipMalloc(&g->d_sync, sizeof(unsigned) * (R + 1)));
+      HIPCHK(hipMemsetAsync(g->d_sync, 0, sizeof(unsigned) * (R + 1), g->stream));
+      HIPCHK(hipMalloc(&g->d_xbuf, sizeof(double) * R * 2 * N * XW));
+      HIPCHK(hipHostMalloc(&g->h_abort, sizeof(unsigned), hipHostMallocDefault));
+      *g->h_abort = 0u;
+    }
     return 0;
   }());
   CK(build_wall(g));
@@ -491,6 +583,11 @@ extern "C" int remd_gpu_execute_pending(remd_gpu *g) {
 extern "C" int remd_gpu_execute_steps_plain(remd_gpu *g, const double *T, const double *P) {
   int rc = remd_gpu_set_conditions(g, T, P);
   if (rc) return rc;
+  // one replica per launch, one launch after another; a resident replica
+  // takes as many workgroups as it can use
+  const int parts = g->resident ? pick_parts(g->capacity, 1, g->N) : 1;
+  g->last_parts = parts;
+  if (parts > 1) HIPCHK(hipMemsetAsync(g->d_sync, 0, sizeof(unsigned) * (g->nrep + 1), g->stream));
   HIPCHK(hipEventRecord(g->ev0, g->stream));
   for (int r = 0; r < g->nrep; r++) {
     LaunchArgs a;
@@ -499,9 +596,15 @@ extern "C" int remd_gpu_execute_steps_plain(remd_gpu *g, const double *T, const 
     a.mode = MODE_INTERVAL;
     a.do_sort = 0;
     a.margin = 0.0;
-    HIPCHK(launch_interval(a, g->nt, g->NS, g->prm.precision, g->lds, g->stream));
+    a.parts = parts;
+    a.sync = g->d_sync ? g->d_sync + r : nullptr;
+    a.abort = g->d_sync ? g->d_sync + g->nrep : nullptr;
+    if (g->resident) HIPCHK(launch_resident(a, g->NS, g->prm.precision, g->lds, g->stream));
+    else HIPCHK(launch_interval(a, g->nt, g->NS, g->prm.precision, g->lds, g->stream));
   }
   HIPCHK(hipEventRecord(g->ev1, g->stream));
+  if (parts > 1)
+    HIPCHK(hipMemcpyAsync(g->h_abort, g->d_sync + g->nrep, sizeof(unsigned), hipMemcpyDeviceToHost, g->stream));
   rc = sync_scalars(g);
   if (rc) return rc;
   float ms = 0.f;
@@ -579,9 +682,58 @@ extern "C" int remd_gpu_launch_shape(remd_gpu *g, int *bt, int *ch, int *lds) {
   return 0;
 }
 
+extern "C" int remd_gpu_launch_path(remd_gpu *g, int *path, int *parts) {
+  if (!g) return fail(REMD_EINVAL, "null handle");
+  if (path) *path = g->resident ? REMD_PATH_RESIDENT : REMD_PATH_GLOBAL;
+  if (parts) *parts = g->last_parts;
+  return 0;
+}
+
+// Time `reps` force passes (after one warm-up) with the current shape.
+static int time_force_passes(remd_gpu *g, int reps, double *ms_sum) {
+  double tsum = 0;
+  for (int k = 0; k <= reps; k++) {
+    int rc = launch(g, MODE_FORCE, 1, g->prm.margin, g->d_reps, g->nrep, true);
+    if (rc) return rc;
+    rc = finish(g);
+    if (rc) return rc;
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, g->ev0, g->ev1));
+    if (k > 0) tsum += ms;
+  }
+  *ms_sum = tsum;
+  return 0;
+}
+
+static int autotune_resident(remd_gpu *g, int reps) {
+  const int c0 = g->prm.chunks;
+  const int cands[6] = {c0 / 2, c0 - 1, c0, c0 + 1, c0 + 2, 2 * c0};
+  double best = 1e300;
+  int bc = c0;
+  for (int k = 0; k < 6; k++) {
+    const int C = cands[k];
+    if (C < 1 || C > 64 || (k > 0 && C == cands[k - 1])) continue;
+    g->prm.chunks = C;
+    HIPCHK(hipMemcpyAsync(g->d_job, &g->prm, sizeof(JobParams), hipMemcpyHostToDevice, g->stream));
+    double t = 0;
+    int rc = time_force_passes(g, reps, &t);
+    if (rc) return rc;
+    if (t < best) {
+      best = t;
+      bc = C;
+    }
+  }
+  g->prm.chunks = bc;
+  HIPCHK(hipMemcpyAsync(g->d_job, &g->prm, sizeof(JobParams), hipMemcpyHostToDevice, g->stream));
+  int rc = launch(g, MODE_FORCE, 1, g->prm.margin, g->d_reps, g->nrep, false);
+  if (rc) return rc;
+  return sync_scalars(g);
+}
+
 extern "C" int remd_gpu_autotune(remd_gpu *g, int reps) {
   if (!g) return fail(REMD_EINVAL, "null handle");
   if (reps < 1) reps = 3;
+  if (g->resident) return autotune_resident(g, reps);
   const int nts[2] = {128, 256};
   double best = 1e300;
   int bnt = g->nt, bc = g->prm.chunks;
@@ -858,6 +1010,7 @@ extern "C" int remd_batch_create(remd_batch **out) {
   hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&b->ev0);
   if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->evres, hipEventDisableTiming);
   for (int c = 0; c < NCLASS && e == hipSuccess; c++) {
     e = hipStreamCreateWithFlags(&b->cstream[c], hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->cdone[c], hipEventDisableTiming);
@@ -893,14 +1046,23 @@ extern "C" int remd_batch_remove(remd_batch *b, remd_gpu *g) {
 static int rebuild_table(remd_batch *b) {
   std::vector<std::pair<std::pair<int, double>, ReplicaDesc>> rows;
   for (int c = 0; c < NCLASS; c++) b->cls_lds[c] = b->cls_n[c] = b->cls_off[c] = 0;
+  b->res_capacity = 0;
+  b->res_minN = 0;
+  b->res_reps = 0;
   for (remd_gpu *g : b->jobs) {
     refresh_reps(g);
-    const int c = kclass(g->NS, g->prm.precision);
+    const int c = kclass(g->resident, g->NS, g->prm.precision);
     const double w = (double)g->N * g->N * g->NS * g->NS * std::max(1, g->desc.interval);
     for (int r = 0; r < g->nrep; r++) rows.push_back({{c, -w}, g->h_reps[r]});
-    b->cls_lds[c] = std::max(b->cls_lds[c], lds_bytes_host(g->N, g->NS, g->prm.precision, b->nt));
+    b->cls_lds[c] = std::max(b->cls_lds[c], g->lds);
     b->cls_n[c] += g->nrep;
+    if (g->resident) {
+      b->res_capacity = b->res_capacity ? std::min(b->res_capacity, g->capacity) : g->capacity;
+      b->res_minN = b->res_minN ? std::min(b->res_minN, g->N) : g->N;
+      b->res_reps += g->nrep;
+    }
   }
+  b->parts = pick_parts(b->res_capacity, b->res_reps, b->res_minN);
   std::stable_sort(rows.begin(), rows.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
   int off = 0;
   for (int c = 0; c < NCLASS; c++) {
@@ -914,6 +1076,11 @@ static int rebuild_table(remd_batch *b) {
     HIPCHK(hipMalloc(&b->d_table, sizeof(ReplicaDesc) * std::max(n, 1)));
     b->table_cap = n;
   }
+  hipFree(b->d_sync);
+  b->d_sync = nullptr;
+  HIPCHK(hipMalloc(&b->d_sync, sizeof(unsigned) * (n + 1)));
+  if (!b->h_abort) HIPCHK(hipHostMalloc(&b->h_abort, sizeof(unsigned), hipHostMallocDefault));
+  *b->h_abort = 0u;
   std::vector<ReplicaDesc> flat(n);
   for (int k = 0; k < n; k++) flat[k] = rows[k].second;
   if (n) HIPCHK(hipMemcpyAsync(b->d_table, flat.data(), sizeof(ReplicaDesc) * n, hipMemcpyHostToDevice, b->stream));
@@ -934,25 +1101,44 @@ extern "C" int remd_batch_execute(remd_batch *b) {
   }
   // conditions were staged on each job's stream; order them before the launch
   for (remd_gpu *g : b->jobs) HIPCHK(hipStreamSynchronize(g->stream));
+  if (b->parts > 1) HIPCHK(hipMemsetAsync(b->d_sync, 0, sizeof(unsigned) * (b->ntab + 1), b->stream));
   HIPCHK(hipEventRecord(b->ev0, b->stream));
-  for (int c = 0; c < NCLASS; c++) {
-    if (!b->cls_n[c]) continue;
-    if (b->cls_lds[c] > device_lds_limit()) return fail(REMD_EINVAL, "batch LDS exceeds the device limit");
-    HIPCHK(hipStreamWaitEvent(b->cstream[c], b->ev0, 0));
-    LaunchArgs a;
-    a.reps = b->d_table + b->cls_off[c];
-    a.nrep = b->cls_n[c];
-    a.mode = MODE_INTERVAL;
-    a.do_sort = 0;
-    a.margin = 0.0;
-    HIPCHK(launch_interval(a, b->nt, 3 + c / 2, c % 2, b->cls_lds[c], b->cstream[c]));
-    HIPCHK(hipEventRecord(b->cdone[c], b->cstream[c]));
-    HIPCHK(hipStreamWaitEvent(b->stream, b->cdone[c], 0));
+  // resident classes first (concurrently), then the global-state classes
+  for (int pass = 0; pass < 2; pass++) {
+    if (pass == 1) HIPCHK(hipEventRecord(b->evres, b->stream));
+    for (int c = 0; c < NCLASS; c++) {
+      const bool res = c >= 6;
+      if (!b->cls_n[c] || res != (pass == 0)) continue;
+      if (b->cls_lds[c] > device_lds_limit()) return fail(REMD_EINVAL, "batch LDS exceeds the device limit");
+      HIPCHK(hipStreamWaitEvent(b->cstream[c], pass == 0 ? b->ev0 : b->evres, 0));
+      LaunchArgs a;
+      a.reps = b->d_table + b->cls_off[c];
+      a.nrep = b->cls_n[c];
+      a.mode = MODE_INTERVAL;
+      a.do_sort = 0;
+      a.margin = 0.0;
+      a.parts = res ? b->parts : 1;
+      a.sync = b->d_sync + b->cls_off[c];
+      a.abort = b->d_sync + b->ntab;
+      const int nsite = 3 + (c % 6) / 2, prec = c % 2;
+      if (res) HIPCHK(launch_resident(a, nsite, prec, b->cls_lds[c], b->cstream[c]));
+      else HIPCHK(launch_interval(a, b->nt, nsite, prec, b->cls_lds[c], b->cstream[c]));
+      HIPCHK(hipEventRecord(b->cdone[c], b->cstream[c]));
+      HIPCHK(hipStreamWaitEvent(b->stream, b->cdone[c], 0));
+    }
   }
   HIPCHK(hipEventRecord(b->ev1, b->stream));
-  for (remd_gpu *g : b->jobs)
+  if (b->parts > 1)
+    HIPCHK(hipMemcpyAsync(b->h_abort, b->d_sync + b->ntab, sizeof(unsigned), hipMemcpyDeviceToHost, b->stream));
+  for (remd_gpu *g : b->jobs) {
+    g->last_parts = g->resident ? b->parts : 1;
     HIPCHK(hipMemcpyAsync(g->h_sc, g->d_sc, sizeof(double) * NSC * g->nrep, hipMemcpyDeviceToHost, b->stream));
+  }
   HIPCHK(hipStreamSynchronize(b->stream));
+  if (b->parts > 1 && *b->h_abort != 0u) {
+    *b->h_abort = 0u;
+    return fail(REMD_EHIP, "resident batch launch aborted: the parts of a replica were not co-resident");
+  }
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
   b->last_ms = ms;
@@ -969,6 +1155,9 @@ extern "C" void remd_batch_destroy(remd_batch *b) {
   if (!b) return;
   if (b->stream) hipStreamSynchronize(b->stream);
   hipFree(b->d_table);
+  hipFree(b->d_sync);
+  if (b->h_abort) hipHostFree(b->h_abort);
+  if (b->evres) hipEventDestroy(b->evres);
   for (int c = 0; c < NCLASS; c++) {
     if (b->cdone[c]) hipEventDestroy(b->cdone[c]);
     if (b->cstream[c]) hipStreamDestroy(b->cstream[c]);

@@ -28,6 +28,29 @@ namespace remd {
 
 constexpr int WAVE = 64;
 
+// Pointers read out of ReplicaDesc would be generic to the compiler; the
+// device view below carries their address spaces in the loaded types, so
+// job parameters load through the scalar unit (constant) and state arrays
+// with global rather than flat instructions.
+#define REMD_AS_CONST __attribute__((address_space(4)))
+#define REMD_AS_GLOBAL __attribute__((address_space(1)))
+struct DescD {
+  const REMD_AS_CONST JobParams *job;
+  REMD_AS_GLOBAL double *mol;
+  REMD_AS_GLOBAL int *mid;
+  REMD_AS_GLOBAL double *sc;
+  REMD_AS_GLOBAL double *site_f;
+  REMD_AS_GLOBAL int *win;
+  REMD_AS_GLOBAL double *xbuf;
+};
+static_assert(sizeof(DescD) == sizeof(ReplicaDesc), "device view of ReplicaDesc");
+template <class T>
+__device__ __forceinline__ T *gen(REMD_AS_GLOBAL T *p) {
+  return (T *)p;
+}
+typedef const REMD_AS_CONST JobParams &JRef;
+
+
 // Optional phase timers (constant 100 MHz clock), accumulated by thread 0
 // into S_DIAG0.. of the replica's scalar block; compiled out by default.
 #ifdef REMD_PHASE_TIMERS
@@ -35,7 +58,7 @@ constexpr int WAVE = 64;
 #define PT_MARK(k, sc)                                        \
   do {                                                        \
     const uint64_t pt_t1 = wall_clock64();                    \
-    if (threadIdx.x == 0) (sc)[S_DIAG0 + (k)] += (double)(pt_t1 - pt_t0); \
+    if (threadIdx.x == 0 && (sc)) (sc)[S_DIAG0 + (k)] += (double)(pt_t1 - pt_t0); \
     pt_t0 = pt_t1;                                            \
   } while (0)
 #else
@@ -143,7 +166,7 @@ __device__ inline void store_sc(const Sc &c, double *sc) {
 }
 
 // Molecules::GetVolume / GetBottomArea (src/molecules.h:357-376)
-__device__ inline double volume(const JobParams &J, double L0, double L1, double L2) {
+__device__ inline double volume(JRef J, double L0, double L1, double L2) {
   if (J.cmode == 1) return J.bottom_area * L2;
   if (J.cmode == 2) return L0 * L1 * J.wall_length;
   return L0 * L1 * L2;
@@ -198,10 +221,10 @@ __device__ inline double rotate(double *q, double *p, int axis, double coef_xi, 
 }
 
 // ------------------------------------------------------------- integrator
-__device__ inline int teff(const JobParams &J) { return J.tmode == 1 ? 1 : 0; }
+__device__ inline int teff(JRef J) { return J.tmode == 1 ? 1 : 0; }
 
 // D6 (src/integrator.cxx:634-644)
-__device__ inline void D6(const JobParams &J, Sc &c) {
+__device__ inline void D6(JRef J, Sc &c) {
   if (teff(J)) {
     const double tmp = 1.0 + (c.Ps / (2.0 * c.Q) * J.dthalf);
     c.s *= tmp * tmp;
@@ -211,7 +234,7 @@ __device__ inline void D6(const JobParams &J, Sc &c) {
 
 // scalar part of D5 (src/integrator.cxx:581-600); pconstant 3 is the area
 // barostat (Lx = Ly = sqrt(A), conjugate momentum in Pv[0]).
-__device__ inline void D5_scalars(const JobParams &J, Sc &c) {
+__device__ inline void D5_scalars(JRef J, Sc &c) {
   if (teff(J)) {
     c.Ps -= c.pot * J.dthalf;
     if (J.pmode > 0) c.Ps -= (c.P * volume(J, c.L[0], c.L[1], c.L[2])) * J.dthalf;
@@ -227,7 +250,7 @@ __device__ inline void D5_scalars(const JobParams &J, Sc &c) {
 
 // D4 (src/integrator.cxx:604-632; pconstant 1 with the Pv^2/(2W) kinetic
 // term of remdgpu.cu:2090-2095)
-__device__ inline void D4(const JobParams &J, Sc &c) {
+__device__ inline void D4(JRef J, Sc &c) {
   if (J.pmode == 1) {
     if (teff(J)) c.Ps -= 0.5 * c.Pv[0] * c.Pv[0] / c.W * J.dthalf;
     const double V = volume(J, c.L[0], c.L[1], c.L[2]) + c.Pv[0] * c.s / c.W * J.dthalf;
@@ -248,7 +271,7 @@ __device__ inline void D4(const JobParams &J, Sc &c) {
 
 // D5 on one molecule (src/integrator.cxx:564-580): v += f L s dt/2 / m,
 // p += S(q) n 2 s dt/2.
-__device__ inline void D5_mol(const JobParams &J, const Sc &c, double *v, const double *f,
+__device__ inline void D5_mol(JRef J, const Sc &c, double *v, const double *f,
                               double *p, const double *q, const double *n) {
   const double m = J.mass;
   v[0] += f[0] * (c.L[0] * c.s * J.dthalf) / m;
@@ -268,10 +291,10 @@ __device__ inline void D5_mol(const JobParams &J, const Sc &c, double *v, const 
 // ------------------------------------------------------------ properties
 // Molecules::CalcProperties (src/molecules.cxx:702-734) over the replica.
 template <int NT>
-__device__ void calc_properties(const JobParams &J, const double *mol, int N, Sc &c, double *sc,
+__device__ void calc_properties(JRef J, const double *mol, int N, Sc &c, double *sc,
                                 double *red) {
   double acc[4] = {0, 0, 0, 0};
-  const double *I = J.inertia;
+  const double *I = (const double *)J.inertia;
   for (int i = threadIdx.x; i < N; i += NT) {
     const double vx = mol[F_VX * N + i], vy = mol[F_VY * N + i], vz = mol[F_VZ * N + i];
     acc[0] += (vx * vx) * J.mass;
@@ -310,7 +333,7 @@ __device__ void calc_properties(const JobParams &J, const double *mol, int N, Sc
   if (J.tmode == 1) tot += tsm + tsp;
   if (J.pmode > 0) tot += bst;
   const double V = volume(J, c.L[0], c.L[1], c.L[2]);
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0 && sc) {
     sc[S_TRA0] = tra[0]; sc[S_TRA1] = tra[1]; sc[S_TRA2] = tra[2];
     sc[S_TRA] = tr; sc[S_ROT] = rot; sc[S_KIN] = kin;
     sc[S_TSM] = tsm; sc[S_TSP] = tsp; sc[S_TST] = tsm + tsp; sc[S_BST] = bst;
@@ -326,9 +349,9 @@ __device__ void calc_properties(const JobParams &J, const double *mol, int N, Sc
 
 // Returns tot (same value on every thread) without writing sc.
 template <int NT>
-__device__ double total_energy(const JobParams &J, const double *mol, int N, const Sc &c, double *red) {
+__device__ double total_energy(JRef J, const double *mol, int N, const Sc &c, double *red) {
   double acc[4] = {0, 0, 0, 0};
-  const double *I = J.inertia;
+  const double *I = (const double *)J.inertia;
   for (int i = threadIdx.x; i < N; i += NT) {
     const double vx = mol[F_VX * N + i], vy = mol[F_VY * N + i], vz = mol[F_VZ * N + i];
     acc[0] += (vx * vx) * J.mass;
@@ -365,7 +388,7 @@ __device__ double total_energy(const JobParams &J, const double *mol, int N, con
 
 // KillMomentum + KillAngularMomentumInTube (src/molecules.cxx:168-210)
 template <int NT>
-__device__ void kill_momentum(const JobParams &J, double *mol, int N, const Sc &c, double *red,
+__device__ void kill_momentum(JRef J, double *mol, int N, const Sc &c, double *red,
                               bool tube_twice) {
   double acc[4] = {0, 0, 0, 0};
   for (int i = threadIdx.x; i < N; i += NT) {
@@ -417,7 +440,7 @@ __device__ void kill_momentum(const JobParams &J, double *mol, int N, const Sc &
 
 // VelocityScaling + AngVelocityScaling (src/molecules.cxx:261-311)
 template <int NT>
-__device__ void velocity_scaling(const JobParams &J, double *mol, int N, Sc &c, double *red) {
+__device__ void velocity_scaling(JRef J, double *mol, int N, Sc &c, double *red) {
   double acc[3] = {0, 0, 0};
   for (int i = threadIdx.x; i < N; i += NT) {
     acc[0] += mol[F_VX * N + i] * mol[F_VX * N + i] * J.mass;
@@ -446,7 +469,7 @@ __device__ void velocity_scaling(const JobParams &J, double *mol, int N, Sc &c, 
   }
   // AngVelocityScaling
   double rr[1] = {0.0};
-  const double *I = J.inertia;
+  const double *I = (const double *)J.inertia;
   for (int i = threadIdx.x; i < N; i += NT) {
     const double q0 = mol[F_Q0 * N + i], q1 = mol[F_Q1 * N + i], q2 = mol[F_Q2 * N + i], q3 = mol[F_Q3 * N + i];
     const double p0 = mol[F_P0 * N + i], p1 = mol[F_P1 * N + i], p2 = mol[F_P2 * N + i], p3 = mol[F_P3 * N + i];
@@ -592,7 +615,7 @@ __device__ inline bool window_full(const double *z, int i, double w, int js, int
 // state array, rebuild js/je (CalcForcePot's doSort branch,
 // src/molecules.cxx:543-585).
 template <int NT>
-__device__ void sort_and_window(const JobParams &J, double *mol, int *mid, int N, double Laxis,
+__device__ void sort_and_window(JRef J, double *mol, int *mid, int N, double Laxis,
                                 double margin, const Lds &S, int *win_dbg) {
   uint64_t *keyA = (uint64_t *)S.region0;
   uint64_t *keyB = keyA + N;
@@ -757,7 +780,7 @@ __device__ __forceinline__ void pair_acc(const PT *gi, const PT (&ai)[NS][3], co
 
 // M2A: LDS records (COM centred on the box, site offsets R(q) b).
 template <int NT, int NS, typename PT>
-__device__ void m2a(const JobParams &J, const double *mol, int N, double L0, double L1, double L2, PT *rec) {
+__device__ void m2a(JRef J, const double *mol, int N, double L0, double L1, double L2, PT *rec) {
   constexpr int ST = Rec<NS>::STRIDE;
   for (int i = threadIdx.x; i < N; i += NT) {
     PT *o = rec + i * ST;
@@ -768,7 +791,7 @@ __device__ void m2a(const JobParams &J, const double *mol, int N, double L0, dou
 #pragma unroll
     for (int a = 0; a < NS; a++) {
       double t[3];
-      body_to_space(q, J.site_r[a], t);
+      body_to_space(q, (const double *)J.site_r[a], t);
       o[3 + 3 * a] = (PT)t[0];
       o[4 + 3 * a] = (PT)t[1];
       o[5 + 3 * a] = (PT)t[2];
@@ -779,7 +802,7 @@ __device__ void m2a(const JobParams &J, const double *mol, int N, double L0, dou
 // Pair pass + wall + A2M for every molecule of the replica.  Writes f, n
 // into the molecule block and the energy/virial totals into c.
 template <int NT, int NS, typename PT>
-__device__ void force_pass(const JobParams &J, double *mol, int N, Sc &c, const Lds &S,
+__device__ void force_pass(JRef J, double *mol, int N, Sc &c, const Lds &S,
                            double *site_dbg, double *pt_sc) {
   PT_DECL
   constexpr int QB = (NS == 3) ? 0 : 1;
@@ -895,7 +918,7 @@ __device__ void force_pass(const JobParams &J, double *mol, int N, Sc &c, const 
       }
       if (J.cmode == 1 || J.cmode == 2) {
         double b0[3];
-        body_to_space(q, J.site_r[0], b0);
+        body_to_space(q, (const double *)J.site_r[0], b0);
         if (J.cmode == 1) {
           const double x = (mol[F_RX * N + i] - 0.5) * c.L[0] + b0[0];
           const double y = (mol[F_RY * N + i] - 0.5) * c.L[1] + b0[1];
@@ -927,7 +950,7 @@ __device__ void force_pass(const JobParams &J, double *mol, int N, Sc &c, const 
       for (int a = 0; a < NS; a++) {
         double fb[3];
         space_to_body(q, acc.site[a], fb);
-        const double *br = J.site_r[a];
+        const double *br = (const double *)J.site_r[a];
         n4[0] += br[0] * fb[0] + br[1] * fb[1] + br[2] * fb[2];
         n4[1] += br[1] * fb[2] - br[2] * fb[1];
         n4[2] += br[2] * fb[0] - br[0] * fb[2];
@@ -967,11 +990,10 @@ __device__ void force_pass(const JobParams &J, double *mol, int N, Sc &c, const 
 }
 
 // ----------------------------------------------------------- one step
-template <int NT, int NS, typename PT>
-__device__ void md_step(const JobParams &J, double *mol, int *mid, int N, Sc &c, const Lds &S,
-                        bool do_sort, const ReplicaDesc &R) {
-  PT_DECL
-  // D6, D5 scalars, D4 (src/integrator.cxx:651-658)
+// First half of ExecuteStep (src/integrator.cxx:651-678): D6, D5 scalars,
+// D4, then per molecule D5 (old box), D3 D2 D1 D2 D3 (new box), PsPv, D4.
+template <int NT>
+__device__ void first_half(JRef J, double *mol, int N, Sc &c, double *red) {
   D6(J, c);
   Sc c_old;  // D5 on the molecules uses the box and s from before D4
   c_old.s = c.s;
@@ -980,7 +1002,6 @@ __device__ void md_step(const JobParams &J, double *mol, int *mid, int N, Sc &c,
   c_old.L[2] = c.L[2];
   D5_scalars(J, c);
   D4(J, c);
-  // per molecule: D5 (old box), D3 D2 D1 D2 D3 (new box)
   const bool T1 = teff(J) == 1;
   const bool needPv = (J.pmode > 0) || T1;
   double sums[4] = {0, 0, 0, 0};  // sum_Pv[3], sum_Ps
@@ -988,12 +1009,12 @@ __device__ void md_step(const JobParams &J, double *mol, int *mid, int N, Sc &c,
     const double coef_xi = 0.25 / c.s;
     const double si = 1.0 / c.s;
     double coef_r[3];
-  #pragma unroll
-  for (int d = 0; d < 3; d++) {
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
       const double Li = 1.0 / c.L[d];
       coef_r[d] = Li * Li * si * J.dt;
     }
-    const double *I = J.inertia;
+    const double *I = (const double *)J.inertia;
     for (int i = threadIdx.x; i < N; i += NT) {
       double r[3] = {mol[F_RX * N + i], mol[F_RY * N + i], mol[F_RZ * N + i]};
       double v[3] = {mol[F_VX * N + i], mol[F_VY * N + i], mol[F_VZ * N + i]};
@@ -1008,7 +1029,7 @@ __device__ void md_step(const JobParams &J, double *mol, int *mid, int N, Sc &c,
       xi = rotate(q, p, 1, coef_xi, I[1], J.dthalf);
       if (T1) sums[3] += 0.5 * xi * xi * I[1];
 #pragma unroll
-  for (int d = 0; d < 3; d++) {
+      for (int d = 0; d < 3; d++) {
         r[d] += v[d] * coef_r[d];
         if (r[d] < 0.0) r[d] += 1.0;
         if (r[d] >= 1.0) r[d] -= 1.0;
@@ -1031,7 +1052,7 @@ __device__ void md_step(const JobParams &J, double *mol, int *mid, int N, Sc &c,
     }
   }
 
-  block_sum<NT, 4>(sums, S.red);  // includes __syncthreads: molecule writes visible
+  block_sum<NT, 4>(sums, red);  // includes __syncthreads: molecule writes visible
   // PsPv (src/integrator.cxx:145-171)
   if (needPv) {
     for (int d = 0; d < 3; d++) sums[d] /= c.L[d] * c.L[d] * c.s * c.s;
@@ -1044,13 +1065,12 @@ __device__ void md_step(const JobParams &J, double *mol, int *mid, int N, Sc &c,
   }
   if (T1) c.Ps += (0.25 * (sums[0] + sums[1] + sums[2]) + 2.0 * sums[3] - (1.0 + log(c.s)) * c.gkT + c.H0) * J.dt;
   D4(J, c);
-  PT_MARK(0, R.sc);
-  // force
-  if (do_sort) sort_and_window<NT>(J, mol, mid, N, J.sort_axis == 0 ? c.L[0] : c.L[2], J.margin, S, R.win);
-  PT_MARK(1, R.sc);
-  force_pass<NT, NS, PT>(J, mol, N, c, S, R.site_f, R.sc);  // ends with block_sum (synced)
-  PT_MARK(2, R.sc);
-  // D5, D6
+}
+
+// Second half (src/integrator.cxx:686-703): D5 on every molecule with the
+// new forces, D5 scalars, D6.  Ends with a workgroup barrier.
+template <int NT>
+__device__ void second_half(JRef J, double *mol, int N, Sc &c) {
   for (int i = threadIdx.x; i < N; i += NT) {
     double v[3] = {mol[F_VX * N + i], mol[F_VY * N + i], mol[F_VZ * N + i]};
     const double f[3] = {mol[F_FX * N + i], mol[F_FY * N + i], mol[F_FZ * N + i]};
@@ -1064,39 +1084,53 @@ __device__ void md_step(const JobParams &J, double *mol, int *mid, int N, Sc &c,
   D5_scalars(J, c);
   D6(J, c);
   __syncthreads();
-  PT_MARK(3, R.sc);
+}
+
+template <int NT, int NS, typename PT>
+__device__ void md_step(JRef J, double *mol, int *mid, int N, Sc &c, const Lds &S,
+                        bool do_sort, const DescD &R) {
+  PT_DECL
+  first_half<NT>(J, mol, N, c, S.red);
+  PT_MARK(0, gen(R.sc));
+  // force
+  if (do_sort) sort_and_window<NT>(J, mol, mid, N, J.sort_axis == 0 ? c.L[0] : c.L[2], J.margin, S, gen(R.win));
+  PT_MARK(1, gen(R.sc));
+  force_pass<NT, NS, PT>(J, mol, N, c, S, gen(R.site_f), gen(R.sc));  // ends with block_sum (synced)
+  PT_MARK(2, gen(R.sc));
+  second_half<NT>(J, mol, N, c);
+  PT_MARK(3, gen(R.sc));
 }
 
 // --------------------------------------------------------- replica driver
 template <int NT, int NS, typename PT>
-__device__ void run_replica(const LaunchArgs &a, const ReplicaDesc &R, const JobParams &J, char *smem) {
+__device__ void run_replica(const LaunchArgs &a, const DescD &R, JRef J, char *smem) {
   const int N = J.nmol;
-  double *mol = R.mol;
-  int *mid = R.mid;
+  double *mol = gen(R.mol);
+  int *mid = gen(R.mid);
   Lds S = lds_carve<NT>(smem, N, NS, J.precision);
   Sc c;
-  load_sc(c, R.sc);
+  load_sc(c, gen(R.sc));
 
   if (a.mode == MODE_INIT) {
     // InitializeProperty (src/molecules.h:260-270): sort with margin 0,
     // force, gkT, CalcProperties, H0 = tot.
-    sort_and_window<NT>(J, mol, mid, N, J.sort_axis == 0 ? c.L[0] : c.L[2], 0.0, S, R.win);
-    force_pass<NT, NS, PT>(J, mol, N, c, S, R.site_f, R.sc);
+    sort_and_window<NT>(J, mol, mid, N, J.sort_axis == 0 ? c.L[0] : c.L[2], 0.0, S, gen(R.win));
+    force_pass<NT, NS, PT>(J, mol, N, c, S, gen(R.site_f), gen(R.sc));
     c.gkT = 6.0 * (double)N * c.T;
     const double tot = total_energy<NT>(J, mol, N, c, S.red);
     c.H0 = tot;
-    calc_properties<NT>(J, mol, N, c, R.sc, S.red);
-    if (threadIdx.x == 0) store_sc(c, R.sc);
+    calc_properties<NT>(J, mol, N, c, gen(R.sc), S.red);
+    if (threadIdx.x == 0) store_sc(c, gen(R.sc));
     return;
   }
   if (a.mode == MODE_FORCE) {
-    if (a.do_sort) sort_and_window<NT>(J, mol, mid, N, J.sort_axis == 0 ? c.L[0] : c.L[2], a.margin, S, R.win);
+    if (a.do_sort) sort_and_window<NT>(J, mol, mid, N, J.sort_axis == 0 ? c.L[0] : c.L[2], a.margin, S, gen(R.win));
     else {
       // reuse the stored window
       for (int i = threadIdx.x; i < N; i += NT) {
-        int js = R.win ? R.win[3 * i] : 0;
-        int je = R.win ? R.win[3 * i + 1] : N - 1;
-        const bool full = R.win ? R.win[3 * i + 2] != 0 : true;
+        int js = gen(R.win) ? gen(R.win)[3 * i] : 0;
+        int je = gen(R.win) ? gen(R.win)[3 * i + 1] : N - 1;
+        const bool full = gen(R.win) ? gen(R.win)[3 * i + 2] != 0 : true;
         if (js >= je) je += N;
         if (full || !((js <= i && i <= je) || (js <= i + N && i + N <= je))) {
           js = i + 1;
@@ -1107,19 +1141,19 @@ __device__ void run_replica(const LaunchArgs &a, const ReplicaDesc &R, const Job
       }
       __syncthreads();
     }
-    force_pass<NT, NS, PT>(J, mol, N, c, S, R.site_f, R.sc);
-    calc_properties<NT>(J, mol, N, c, R.sc, S.red);
-    if (threadIdx.x == 0) store_sc(c, R.sc);
+    force_pass<NT, NS, PT>(J, mol, N, c, S, gen(R.site_f), gen(R.sc));
+    calc_properties<NT>(J, mol, N, c, gen(R.sc), S.red);
+    if (threadIdx.x == 0) store_sc(c, gen(R.sc));
     return;
   }
 
   // MODE_INTERVAL
-  const double xflag = R.sc[S_XFLAG];
+  const double xflag = gen(R.sc)[S_XFLAG];
   if (xflag != 0.0) {
     // REMD::ReplicaExchange for an exchanged replica
     // (src/remd.cxx:526-536): ChangeTemperature, KillMomentum, SetPressure,
     // CalcProperties, UpdateHamiltonian.
-    const double Tn = R.sc[S_TNEW];
+    const double Tn = gen(R.sc)[S_TNEW];
     const double coef = sqrt(Tn / c.T);
     for (int i = threadIdx.x; i < N; i += NT) {
       for (int d = 0; d < 3; d++) mol[(F_VX + d) * N + i] *= coef / c.s;
@@ -1134,7 +1168,7 @@ __device__ void run_replica(const LaunchArgs &a, const ReplicaDesc &R, const Job
     c.s = 1.0;
     __syncthreads();
     kill_momentum<NT>(J, mol, N, c, S.red, false);
-    c.P = R.sc[S_PNEW];
+    c.P = gen(R.sc)[S_PNEW];
     c.H0 = total_energy<NT>(J, mol, N, c, S.red);
   }
   if (J.tmode == 2) velocity_scaling<NT>(J, mol, N, c, S.red);
@@ -1143,11 +1177,11 @@ __device__ void run_replica(const LaunchArgs &a, const ReplicaDesc &R, const Job
     const bool do_sort = (st % J.sort_interval) == 0;
     md_step<NT, NS, PT>(J, mol, mid, N, c, S, do_sort, R);
   }
-  calc_properties<NT>(J, mol, N, c, R.sc, S.red);
+  calc_properties<NT>(J, mol, N, c, gen(R.sc), S.red);
   if (threadIdx.x == 0) {
-    store_sc(c, R.sc);
-    R.sc[S_XFLAG] = 0.0;
-    R.sc[S_STEPS] += (double)J.interval;
+    store_sc(c, gen(R.sc));
+    gen(R.sc)[S_XFLAG] = 0.0;
+    gen(R.sc)[S_STEPS] += (double)J.interval;
   }
 }
 
@@ -1155,9 +1189,570 @@ template <int NT, int NS, typename PT>
 __global__ __launch_bounds__(NT) void interval_kernel(LaunchArgs a) {
   extern __shared__ __align__(16) char smem[];
   if ((int)blockIdx.x >= a.nrep) return;
-  const ReplicaDesc R = a.reps[blockIdx.x];
-  const JobParams &J = *R.job;
+  const DescD R = ((const DescD *)a.reps)[blockIdx.x];
+  JRef J = *R.job;
   run_replica<NT, NS, PT>(a, R, J, smem);
+}
+
+// ====================================================== replica-resident path
+// For replicas whose state fits in LDS.  The replica's molecule arrays are
+// loaded into LDS at the start of a launch and written back at its end.  A
+// replica may be split over G workgroups ("parts"): every part integrates
+// all molecules (the same instructions on the same data, so the copies stay
+// bit-identical) and computes the forces of a contiguous molecule range; the
+// parts exchange per-molecule forces, torques, energies and virials through
+// global memory once per step behind a group barrier.  Each molecule's force
+// is computed whole by one part with the job's fixed chunk count, and the
+// energies are reduced from per-molecule values in a fixed order, so results
+// do not depend on G, on batching or on placement.
+
+constexpr int XW = 16;                 // doubles per molecule row of the force exchange
+constexpr unsigned MAXSPIN = 1u << 20; // group-barrier polls before the launch gives up
+
+template <int NS>
+struct NVal {
+  static constexpr int V = 3 * NS + 8;  // site forces, COM correction, virial, lj, clmb
+};
+
+struct LdsR {
+  double *mol;      // NFIELD x N replica state
+  int *mid;         // N molecule identities
+  char *region0;    // site records, or the sort scratch
+  uint16_t *js, *je;
+  double *part;     // NV x NT pair partials, value-major
+  double *E;        // 6 x N per-molecule lj, clmb, wall, vir[3]
+  double *red;      // block_sum scratch
+  int *flag;        // barrier status broadcast
+};
+
+__host__ __device__ inline int resident_region0(int nmol, int nsite, int prec) {
+  const int stride = (nsite == 5) ? 18 : (nsite == 4 ? 16 : 12);
+  const int rec = nmol * stride * (prec ? 8 : 4);
+  const int sortb = align16(10 * nmol) + 8 * nmol;  // keys, ranks, one staged field
+  return align16(rec > sortb ? rec : sortb);
+}
+
+// Bytes of dynamic LDS the resident path needs for one replica.
+__host__ __device__ inline int resident_lds_bytes(int nmol, int nsite, int prec, int nt) {
+  return align16(NFIELD * 8 * nmol) + align16(4 * nmol) + resident_region0(nmol, nsite, prec) +
+         align16(4 * nmol) + (3 * nsite + 8) * 8 * nt + align16(6 * 8 * nmol) + (nt / WAVE) * 32 * 8 + 64;
+}
+
+template <int NT, int NS>
+__device__ inline LdsR carve_resident(char *smem, int N, int prec) {
+  LdsR L;
+  char *p = smem;
+  L.mol = (double *)p;
+  p += align16(NFIELD * 8 * N);
+  L.mid = (int *)p;
+  p += align16(4 * N);
+  L.region0 = p;
+  p += resident_region0(N, NS, prec);
+  L.js = (uint16_t *)p;
+  L.je = L.js + N;
+  p += align16(4 * N);
+  L.part = (double *)p;
+  p += NVal<NS>::V * 8 * NT;
+  L.E = (double *)p;
+  p += align16(6 * 8 * N);
+  L.red = (double *)p;
+  p += (NT / WAVE) * 32 * 8;
+  L.flag = (int *)p;
+  return L;
+}
+
+struct Group {
+  int parts, part;
+  unsigned *ctr;    // arrivals of this replica's parts, zero at launch start
+  unsigned *abort;  // nonzero once any wait of the launch timed out
+  double *xbuf;     // 2 x N x XW exchange rows (double-buffered by step parity)
+  unsigned epoch;   // group barriers completed in this launch
+};
+
+// Barrier across the parts of one replica.  Every thread first makes its
+// own global stores visible at device scope; one thread counts in and polls
+// the counter (bounded; a timeout raises the launch-wide abort flag so every
+// part leaves); every thread then acquires before reading what the other
+// parts wrote.  Returns false when the launch is aborting.
+template <int NT>
+__device__ bool group_wait(Group &g, int *flag) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    const unsigned target = (unsigned)g.parts * (g.epoch + 1u);
+    __hip_atomic_fetch_add(g.ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned spins = 0;
+    while (__hip_atomic_load(g.ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      ++spins;
+      if ((spins & 63u) == 0u) {
+        if (__hip_atomic_load(g.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+          ok = 0;
+          break;
+        }
+        if (spins > MAXSPIN) {
+          __hip_atomic_store(g.abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          ok = 0;
+          break;
+        }
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    *flag = ok;
+  }
+  __syncthreads();
+  const bool ok = *flag != 0;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  g.epoch++;
+  __syncthreads();  // flag is rewritten by the next barrier
+  return ok;
+}
+
+// Stable rank sort of the sort-axis coordinate in LDS (the same order as a
+// stable sort on the double keys: rank = #smaller + #equal-and-earlier),
+// permutation of the state, then the windows (src/molecules.cxx:541-585
+// semantics; raw js/je and the whole-ring flag go to win_dbg when given).
+template <int NT>
+__device__ void rank_sort_resident(JRef J, double *mol, int *mid, int N, double Laxis,
+                                   double margin, const LdsR &S, int *win_dbg) {
+  double *z = (double *)S.region0;
+  uint16_t *rk = (uint16_t *)(S.region0 + 8 * N);
+  double *tmp = (double *)(S.region0 + align16(10 * N));
+  double *ax = mol + (F_RX + J.sort_axis) * N;
+  for (int i = threadIdx.x; i < N; i += NT) z[i] = ax[i] + 0.0;  // -0.0 -> +0.0
+  __syncthreads();
+  for (int i = threadIdx.x; i < N; i += NT) {
+    const double zi = z[i];
+    int r = 0;
+    for (int j = 0; j < N; ++j) {
+      const double zj = z[j];
+      r += (zj < zi || (zj == zi && j < i)) ? 1 : 0;
+    }
+    rk[i] = (uint16_t)r;
+  }
+  __syncthreads();
+  for (int f = 0; f < NFIELD; ++f) {
+    double *col = mol + f * N;
+    for (int i = threadIdx.x; i < N; i += NT) tmp[rk[i]] = col[i];
+    __syncthreads();
+    for (int i = threadIdx.x; i < N; i += NT) col[i] = tmp[i];
+    __syncthreads();
+  }
+  {
+    int *ti = (int *)tmp;
+    for (int i = threadIdx.x; i < N; i += NT) ti[rk[i]] = mid[i];
+    __syncthreads();
+    for (int i = threadIdx.x; i < N; i += NT) mid[i] = ti[i];
+    __syncthreads();
+  }
+  const double w = (J.rcut + margin) / Laxis;
+  for (int i = threadIdx.x; i < N; i += NT) {
+    int js, je;
+    window_of(ax, N, i, w, js, je);
+    const bool full = window_full(ax, i, w, js, je);
+    if (win_dbg) {
+      win_dbg[3 * i] = js;
+      win_dbg[3 * i + 1] = je;
+      win_dbg[3 * i + 2] = full ? 1 : 0;
+    }
+    if (js >= je) je += N;
+    if (full || !((js <= i && i <= je) || (js <= i + N && i + N <= je))) {
+      js = i + 1;
+      je = i + N - 1;
+    }
+    S.js[i] = (uint16_t)js;
+    S.je[i] = (uint16_t)je;
+  }
+  __syncthreads();
+}
+
+// Window of the last sort, from the debug copy (MODE_FORCE without a sort).
+template <int NT>
+__device__ void window_from_debug(const int *win, int N, const LdsR &S) {
+  for (int i = threadIdx.x; i < N; i += NT) {
+    int js = win ? win[3 * i] : 0;
+    int je = win ? win[3 * i + 1] : N - 1;
+    const bool full = win ? win[3 * i + 2] != 0 : true;
+    if (js >= je) je += N;
+    if (full || !((js <= i && i <= je) || (js <= i + N && i + N <= je))) {
+      js = i + 1;
+      je = i + N - 1;
+    }
+    S.js[i] = (uint16_t)js;
+    S.je[i] = (uint16_t)je;
+  }
+  __syncthreads();
+}
+
+// Pair pass for molecules [i0, i1) of the replica (switched LJ + Coulomb,
+// src/force_calculator.cxx:803-989 semantics; wall src/force_calculator.cxx:
+// 1494-1531; A2M torque src/molecules.cxx:401-439).  Work item (i, c): chunk
+// c of C runs over i's window with stride C; the C partials of i are summed
+// in chunk order by one thread.  Outputs per molecule: F[3], n[4], lj, clmb,
+// wall, vir[3] -- into xrow (global exchange row) when given, else into the
+// LDS state and S.E.
+template <int NT, int NS, typename PT>
+__device__ void force_range(JRef J, double *mol, int N, const Sc &c, const LdsR &S, int i0,
+                            int i1, double *xrow, double *site_dbg) {
+  constexpr int QB = (NS == 3) ? 0 : 1;
+  constexpr int ST = Rec<NS>::STRIDE;
+  constexpr int NV = NVal<NS>::V;
+  const PT *rec = (const PT *)S.region0;
+  PairK<NS, PT> k;
+  if (sizeof(PT) == 4) {
+    k.rc = J.f_rc;
+    k.rl = J.f_rl;
+    k.rc2 = J.f_rc2;
+    k.rl2 = J.f_rl2;
+    k.coef = J.f_coef;
+    k.coef30 = J.f_coef30;
+    k.sig2 = J.f_sig2;
+    k.eps4 = J.f_eps4;
+    k.eps48 = J.f_eps48;
+#pragma unroll
+    for (int a = 0; a < NS; a++)
+#pragma unroll
+      for (int b = 0; b < NS; b++) k.qq[a][b] = J.f_qq[a][b];
+  } else {
+    k.rc = (PT)J.rcut;
+    k.rl = (PT)J.rswitch;
+    k.rc2 = (PT)(J.rcut * J.rcut);
+    k.rl2 = (PT)(J.rswitch * J.rswitch);
+    const double rlc = J.rswitch - J.rcut;
+    const double coef = 1.0 / (rlc * rlc * rlc * rlc * rlc);
+    k.coef = (PT)coef;
+    k.coef30 = (PT)(30.0 * coef);
+    k.sig2 = (PT)J.lj_sig2;
+    k.eps4 = (PT)J.lj_eps4;
+    k.eps48 = (PT)(12.0 * J.lj_eps4);
+#pragma unroll
+    for (int a = 0; a < NS; a++)
+#pragma unroll
+      for (int b = 0; b < NS; b++) k.qq[a][b] = (PT)(J.q[a] * J.q[b]);
+  }
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    k.L[d] = (PT)c.L[d];
+    k.Li[d] = (PT)(1.0 / c.L[d]);
+    k.per[d] = J.per[d];
+  }
+  const int C = J.chunks;
+  const int B = NT / C;  // molecules per round
+  const double dr = (J.cmode == 1) ? J.wall_length / (double)J.nfwall : 0.5 * J.wall_length / (double)J.nfwall;
+  const double dri = 1.0 / dr;
+  const int t = threadIdx.x;
+  for (int ib = i0; ib < i1; ib += B) {
+    const int ie = (ib + B < i1) ? ib + B : i1;
+    const int items = (ie - ib) * C;
+    if (t < items) {
+      const int li = t / C;
+      const int i = ib + li;
+      const int cc = t - li * C;
+      Acc<NS> acc;
+#pragma unroll
+      for (int a = 0; a < NS; a++) acc.site[a][0] = acc.site[a][1] = acc.site[a][2] = 0.0;
+      acc.corr[0] = acc.corr[1] = acc.corr[2] = 0.0;
+      acc.vir[0] = acc.vir[1] = acc.vir[2] = 0.0;
+      acc.lj = acc.clmb = 0.0;
+      const PT *ri = rec + i * ST;
+      PT gi[3] = {ri[0], ri[1], ri[2]};
+      PT ai[NS][3];
+#pragma unroll
+      for (int a = 0; a < NS; a++) {
+        ai[a][0] = ri[3 + 3 * a];
+        ai[a][1] = ri[4 + 3 * a];
+        ai[a][2] = ri[5 + 3 * a];
+      }
+      const int js = S.js[i];
+      const int je = S.je[i];
+      for (int jj = js + cc; jj <= je; jj += C) {
+        const int j = (jj >= N) ? jj - N : jj;
+        if (j == i) continue;
+        PT rj[3 + 3 * NS];
+        const PT *src = rec + j * ST;
+#pragma unroll
+        for (int u = 0; u < 3 + 3 * NS; u++) rj[u] = src[u];
+        pair_acc<NS, QB, PT>(gi, ai, rj, k, acc);
+      }
+      double *P = S.part;
+#pragma unroll
+      for (int a = 0; a < NS; a++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) P[(3 * a + d) * NT + t] = acc.site[a][d];
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        P[(3 * NS + d) * NT + t] = acc.corr[d];
+        P[(3 * NS + 3 + d) * NT + t] = acc.vir[d];
+      }
+      P[(3 * NS + 6) * NT + t] = acc.lj;
+      P[(3 * NS + 7) * NT + t] = acc.clmb;
+    }
+    __syncthreads();
+    if (t < ie - ib) {
+      const int i = ib + t;
+      const double *P = S.part + t * C;
+      double v[NV];
+#pragma unroll
+      for (int u = 0; u < NV; u++) {
+        double s = P[u * NT];
+        for (int cc = 1; cc < C; cc++) s += P[u * NT + cc];
+        v[u] = s;
+      }
+      double site[NS][3];
+#pragma unroll
+      for (int a = 0; a < NS; a++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) site[a][d] = v[3 * a + d];
+      const double q[4] = {mol[F_Q0 * N + i], mol[F_Q1 * N + i], mol[F_Q2 * N + i], mol[F_Q3 * N + i]};
+      double F[3];
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        double s = v[3 * NS + d];
+#pragma unroll
+        for (int a = 0; a < NS; a++) s += site[a][d];
+        F[d] = s;
+      }
+      double e_wall = 0.0;
+      if (J.cmode == 1 || J.cmode == 2) {
+        double b0[3];
+        body_to_space(q, (const double *)J.site_r[0], b0);
+        if (J.cmode == 1) {
+          const double x = (mol[F_RX * N + i] - 0.5) * c.L[0] + b0[0];
+          const double y = (mol[F_RY * N + i] - 0.5) * c.L[1] + b0[1];
+          const double r01 = sqrt(x * x + y * y);
+          int idx = (int)(r01 / dr);
+          if (idx > J.nfwall - 2) idx = J.nfwall - 2;
+          const double f = (J.fwall[2 * (idx + 1)] - J.fwall[2 * idx]) * dri * (r01 - dr * idx) + J.fwall[2 * idx];
+          const double e = (J.fwall[2 * (idx + 1) + 1] - J.fwall[2 * idx + 1]) * dri * (r01 - dr * idx) + J.fwall[2 * idx + 1];
+          site[0][0] += f * x;
+          site[0][1] += f * y;
+          F[0] += f * x;
+          F[1] += f * y;
+          e_wall = e;
+        } else {
+          const double z = (mol[F_RZ * N + i] - 0.5) * c.L[2] + b0[2];
+          const double az = fabs(z);
+          int idx = (int)(az / dr);
+          if (idx > J.nfwall - 2) idx = J.nfwall - 2;
+          const double f = (J.fwall[2 * (idx + 1)] - J.fwall[2 * idx]) * dri * (az - dr * idx) + J.fwall[2 * idx];
+          const double e = (J.fwall[2 * (idx + 1) + 1] - J.fwall[2 * idx + 1]) * dri * (az - dr * idx) + J.fwall[2 * idx + 1];
+          site[0][2] += f * z;
+          F[2] += f * z;
+          e_wall = e;
+        }
+      }
+      double n4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int a = 0; a < NS; a++) {
+        double fb[3];
+        space_to_body(q, site[a], fb);
+        const double *br = (const double *)J.site_r[a];
+        n4[0] += br[0] * fb[0] + br[1] * fb[1] + br[2] * fb[2];
+        n4[1] += br[1] * fb[2] - br[2] * fb[1];
+        n4[2] += br[2] * fb[0] - br[0] * fb[2];
+        n4[3] += br[0] * fb[1] - br[1] * fb[0];
+      }
+      const double ev[6] = {v[3 * NS + 6], v[3 * NS + 7], e_wall, v[3 * NS + 3], v[3 * NS + 4], v[3 * NS + 5]};
+      if (xrow) {
+        double *o = xrow + (size_t)i * XW;
+        o[0] = F[0];
+        o[1] = F[1];
+        o[2] = F[2];
+        o[3] = n4[0];
+        o[4] = n4[1];
+        o[5] = n4[2];
+        o[6] = n4[3];
+#pragma unroll
+        for (int u = 0; u < 6; u++) o[7 + u] = ev[u];
+      } else {
+        mol[F_FX * N + i] = F[0];
+        mol[F_FY * N + i] = F[1];
+        mol[F_FZ * N + i] = F[2];
+        mol[F_N0 * N + i] = n4[0];
+        mol[F_N1 * N + i] = n4[1];
+        mol[F_N2 * N + i] = n4[2];
+        mol[F_N3 * N + i] = n4[3];
+#pragma unroll
+        for (int u = 0; u < 6; u++) S.E[u * N + i] = ev[u];
+      }
+      if (site_dbg) {
+#pragma unroll
+        for (int a = 0; a < NS; a++)
+#pragma unroll
+          for (int d = 0; d < 3; d++) site_dbg[(i * NS + a) * 3 + d] = site[a][d];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Forces of the whole replica into the LDS state and the energy/virial
+// totals into c: this part's range, the exchange with the other parts,
+// then the fixed-order reduction of the per-molecule values.
+template <int NT, int NS, typename PT>
+__device__ bool forces_resident(JRef J, double *mol, int N, Sc &c, const LdsR &S, Group &g,
+                                int i0, int i1, double *site_dbg, double *pt_sc = nullptr) {
+  PT_DECL
+  (void)pt_sc;
+  m2a<NT, NS, PT>(J, mol, N, c.L[0], c.L[1], c.L[2], (PT *)S.region0);
+  __syncthreads();
+  PT_MARK(4, pt_sc);
+  if (g.parts > 1) {
+    double *xrow = g.xbuf + (size_t)(g.epoch & 1u) * N * XW;
+    force_range<NT, NS, PT>(J, mol, N, c, S, i0, i1, xrow, site_dbg);
+    PT_MARK(5, pt_sc);
+    if (!group_wait<NT>(g, S.flag)) return false;
+    PT_MARK(6, pt_sc);
+    for (int i = threadIdx.x; i < N; i += NT) {
+      const double *o = xrow + (size_t)i * XW;
+      mol[F_FX * N + i] = o[0];
+      mol[F_FY * N + i] = o[1];
+      mol[F_FZ * N + i] = o[2];
+      mol[F_N0 * N + i] = o[3];
+      mol[F_N1 * N + i] = o[4];
+      mol[F_N2 * N + i] = o[5];
+      mol[F_N3 * N + i] = o[6];
+#pragma unroll
+      for (int u = 0; u < 6; u++) S.E[u * N + i] = o[7 + u];
+    }
+    __syncthreads();
+  } else {
+    force_range<NT, NS, PT>(J, mol, N, c, S, 0, N, nullptr, site_dbg);
+    PT_MARK(5, pt_sc);
+  }
+  double tot[6] = {0, 0, 0, 0, 0, 0};
+  for (int i = threadIdx.x; i < N; i += NT) {
+#pragma unroll
+    for (int u = 0; u < 6; u++) tot[u] += S.E[u * N + i];
+  }
+  block_sum<NT, 6>(tot, S.red);
+  c.lj = 0.5 * tot[0];
+  c.clmb = 0.5 * tot[1];
+  c.wall = tot[2];
+  c.vir[0] = 0.5 * tot[3];
+  c.vir[1] = 0.5 * tot[4];
+  c.vir[2] = 0.5 * tot[5];
+  c.pot = c.lj + c.clmb + c.wall;
+  PT_MARK(7, pt_sc);
+  return true;
+}
+
+template <int NT, int NS, typename PT>
+__device__ void run_resident(const LaunchArgs &a, const DescD &R, JRef J, char *smem,
+                             Group &g) {
+  const int N = J.nmol;
+  LdsR S = carve_resident<NT, NS>(smem, N, J.precision);
+  double *mol = S.mol;
+  int *mid = S.mid;
+  for (int k = threadIdx.x; k < NFIELD * N; k += NT) mol[k] = gen(R.mol)[k];
+  for (int i = threadIdx.x; i < N; i += NT) mid[i] = gen(R.mid)[i];
+  Sc c;
+  load_sc(c, gen(R.sc));
+  const double xflag = gen(R.sc)[S_XFLAG];
+  const double Tn = gen(R.sc)[S_TNEW];
+  const double Pn = gen(R.sc)[S_PNEW];
+  __syncthreads();
+  const bool lead = g.part == 0;
+  const int i0 = (int)(((long long)g.part * N) / g.parts);
+  const int i1 = (int)(((long long)(g.part + 1) * N) / g.parts);
+  double *site_dbg = gen(R.site_f);
+  int *win_dbg = lead ? gen(R.win) : nullptr;
+  double *sc_out = lead ? gen(R.sc) : nullptr;
+  const double Laxis0 = J.sort_axis == 0 ? c.L[0] : c.L[2];
+
+  if (a.mode == MODE_INIT) {
+    // InitializeProperty (src/molecules.h:260-270)
+    rank_sort_resident<NT>(J, mol, mid, N, Laxis0, 0.0, S, win_dbg);
+    if (!forces_resident<NT, NS, PT>(J, mol, N, c, S, g, i0, i1, site_dbg)) return;
+    c.gkT = 6.0 * (double)N * c.T;
+    c.H0 = total_energy<NT>(J, mol, N, c, S.red);
+    calc_properties<NT>(J, mol, N, c, sc_out, S.red);
+  } else if (a.mode == MODE_FORCE) {
+    if (a.do_sort) rank_sort_resident<NT>(J, mol, mid, N, Laxis0, a.margin, S, win_dbg);
+    else window_from_debug<NT>(gen(R.win), N, S);
+    if (!forces_resident<NT, NS, PT>(J, mol, N, c, S, g, i0, i1, site_dbg)) return;
+    calc_properties<NT>(J, mol, N, c, sc_out, S.red);
+  } else {
+    if (xflag != 0.0) {
+      // REMD::ReplicaExchange for an exchanged replica (src/remd.cxx:526-536)
+      const double coef = sqrt(Tn / c.T);
+      for (int i = threadIdx.x; i < N; i += NT) {
+        for (int d = 0; d < 3; d++) mol[(F_VX + d) * N + i] *= coef / c.s;
+        for (int d = 0; d < 4; d++) mol[(F_P0 + d) * N + i] *= coef / c.s;
+      }
+      c.Ps *= coef;
+      if (J.pmode == 1) c.Pv[0] *= coef * c.s;
+      if (J.pmode == 2) c.Pv[2] *= coef * c.s;
+      if (J.pmode == 3) c.Pv[0] *= coef * c.s;
+      c.T = Tn;
+      c.gkT = 6.0 * (double)N * Tn;
+      c.s = 1.0;
+      __syncthreads();
+      kill_momentum<NT>(J, mol, N, c, S.red, false);
+      c.P = Pn;
+      c.H0 = total_energy<NT>(J, mol, N, c, S.red);
+    }
+    if (J.tmode == 2) velocity_scaling<NT>(J, mol, N, c, S.red);
+    double *pt = lead ? gen(R.sc) : nullptr;
+    (void)pt;
+    for (int st = 0; st < J.interval; ++st) {
+      PT_DECL
+      if (J.tmode == 2) velocity_scaling<NT>(J, mol, N, c, S.red);
+      first_half<NT>(J, mol, N, c, S.red);
+      PT_MARK(0, pt);
+      if ((st % J.sort_interval) == 0)
+        rank_sort_resident<NT>(J, mol, mid, N, J.sort_axis == 0 ? c.L[0] : c.L[2], J.margin, S, win_dbg);
+      PT_MARK(1, pt);
+      if (!forces_resident<NT, NS, PT>(J, mol, N, c, S, g, i0, i1, site_dbg, pt)) return;
+      PT_MARK(2, pt);
+      second_half<NT>(J, mol, N, c);
+      PT_MARK(3, pt);
+    }
+    calc_properties<NT>(J, mol, N, c, sc_out, S.red);
+  }
+  // every part has read the replica's global state before the lead rewrites it
+  if (g.parts > 1 && !group_wait<NT>(g, S.flag)) return;
+  if (lead) {
+    for (int k = threadIdx.x; k < NFIELD * N; k += NT) gen(R.mol)[k] = mol[k];
+    for (int i = threadIdx.x; i < N; i += NT) gen(R.mid)[i] = mid[i];
+    if (threadIdx.x == 0) {
+      store_sc(c, gen(R.sc));
+      if (a.mode == MODE_INTERVAL) {
+        gen(R.sc)[S_XFLAG] = 0.0;
+        gen(R.sc)[S_STEPS] += (double)J.interval;
+      }
+    }
+  }
+}
+
+// grid: parts == 1 -> one workgroup per replica; parts > 1 -> 8 * parts *
+// ceil(nrep / 8) workgroups, the parts of a replica on block indices that
+// are equal mod 8 (workgroups are dispatched to the 8 XCDs round-robin, so a
+// replica's parts share one L2).
+template <int NT, int NS, typename PT>
+__global__ __launch_bounds__(NT) void resident_kernel(LaunchArgs a) {
+  extern __shared__ __align__(16) char smem[];
+  int rep, part;
+  if (a.parts <= 1) {
+    rep = blockIdx.x;
+    part = 0;
+  } else {
+    const int b = blockIdx.x;
+    const int slot = b >> 3;
+    rep = (slot / a.parts) * 8 + (b & 7);
+    part = slot % a.parts;
+  }
+  if (rep >= a.nrep) return;
+  const DescD R = ((const DescD *)a.reps)[rep];
+  JRef J = *R.job;
+  Group g;
+  g.parts = a.parts < 1 ? 1 : a.parts;
+  g.part = part;
+  g.ctr = a.sync + rep;
+  g.abort = a.abort;
+  g.xbuf = gen(R.xbuf);
+  g.epoch = 0;
+  run_resident<NT, NS, PT>(a, R, J, smem, g);
 }
 
 // ---------------------------------------------------------------- b1
@@ -1339,6 +1934,45 @@ static hipError_t launch_nt(const LaunchArgs &a, int nsite, int prec, int lds, h
   return hipGetLastError();
 }
 
+// Resident path: block size 256 only.
+static const void *resident_fn(int nsite, int prec) {
+  switch (nsite * 2 + prec) {
+    case 6: return (const void *)resident_kernel<256, 3, float>;
+    case 8: return (const void *)resident_kernel<256, 4, float>;
+    case 10: return (const void *)resident_kernel<256, 5, float>;
+    case 7: return (const void *)resident_kernel<256, 3, double>;
+    case 9: return (const void *)resident_kernel<256, 4, double>;
+    case 11: return (const void *)resident_kernel<256, 5, double>;
+    default: return nullptr;
+  }
+}
+
+int resident_grid(int nrep, int parts) { return parts <= 1 ? nrep : 8 * parts * ((nrep + 7) / 8); }
+
+int resident_lds(int nmol, int nsite, int prec) { return resident_lds_bytes(nmol, nsite, prec, 256); }
+
+hipError_t launch_resident(const LaunchArgs &a, int nsite, int prec, int lds, hipStream_t st) {
+  if (a.nrep <= 0) return hipSuccess;
+  const dim3 g(resident_grid(a.nrep, a.parts)), b(256);
+  switch (nsite * 2 + prec) {
+    case 6: hipLaunchKernelGGL((resident_kernel<256, 3, float>), g, b, lds, st, a); break;
+    case 8: hipLaunchKernelGGL((resident_kernel<256, 4, float>), g, b, lds, st, a); break;
+    case 10: hipLaunchKernelGGL((resident_kernel<256, 5, float>), g, b, lds, st, a); break;
+    case 7: hipLaunchKernelGGL((resident_kernel<256, 3, double>), g, b, lds, st, a); break;
+    case 9: hipLaunchKernelGGL((resident_kernel<256, 4, double>), g, b, lds, st, a); break;
+    case 11: hipLaunchKernelGGL((resident_kernel<256, 5, double>), g, b, lds, st, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// Workgroups of the resident kernel one CU holds at once with lds bytes.
+hipError_t resident_occupancy(int nsite, int prec, int lds, int *per_cu) {
+  const void *f = resident_fn(nsite, prec);
+  if (!f) return hipErrorInvalidValue;
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, f, 256, (size_t)lds);
+}
+
 hipError_t launch_interval(const LaunchArgs &a, int nt, int nsite, int prec, int lds, hipStream_t st) {
   if (a.nrep <= 0) return hipSuccess;
   if (nt == 256) return launch_nt<256>(a, nsite, prec, lds, st);
@@ -1365,7 +1999,11 @@ static hipError_t set_lds_nt(int bytes) {
 }
 
 hipError_t set_kernel_lds_limit(int bytes) {
-  hipError_t e = set_lds_nt<256>(bytes);
+  hipError_t e = hipSuccess;
+  for (int ns = 3; ns <= 5 && e == hipSuccess; ns++)
+    for (int pr = 0; pr < 2 && e == hipSuccess; pr++)
+      e = hipFuncSetAttribute(resident_fn(ns, pr), hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  if (e == hipSuccess) e = set_lds_nt<256>(bytes);
   if (e == hipSuccess) e = set_lds_nt<512>(bytes);
   if (e == hipSuccess) e = set_lds_nt<128>(bytes);
   if (e == hipSuccess)

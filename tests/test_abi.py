@@ -174,3 +174,15 @@ def test_product_path_fails_loudly_without_library(monkeypatch, tmp_path):
     monkeypatch.setattr(ra, "LIBPATH", str(tmp_path / "missing.so"))
     with pytest.raises((OSError, RuntimeError, FileNotFoundError)):
         ra.lib()
+
+
+def test_driver_rejects_unsupported_input(tmp_path):
+    """remd_md stops with a message (no device call) on inputs it does not
+    support."""
+    import subprocess
+    drv = os.path.join(ROOT, "remd_amd", "remd_md")
+    for text, word in (("gen_vel 1\nchk_in gpu\n", "gen_vel"), ("ntemp 2\nnreplica 3\nchk_in a\n", "nreplica"),
+                       ("rem_type 1\nchk_in a\n", "designed walk")):
+        (tmp_path / "x.inp").write_text(text)
+        res = subprocess.run([drv, str(tmp_path / "x.inp")], capture_output=True, text=True, timeout=60)
+        assert res.returncode != 0 and word in res.stderr, res.stderr

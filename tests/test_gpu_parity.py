@@ -64,11 +64,26 @@ def test_sort_window_bitexact(n, w, seed):
 
 
 # ------------------------------------------------------------- a1..a5, b2
+PATHS = [ra.PATH_RESIDENT, ra.PATH_GLOBAL]
+
+
+def bulk_large(n=600, L=26.2, seed=41, model="TIP4P", ntemp=2):
+    """A bulk replica too large for the resident path's LDS (n = 600)."""
+    rng = np.random.RandomState(seed)
+    mt = inp.molecule(model)
+    T, P = inp.make_ladder(ntemp, 1, inp.geometric(260.0, 320.0, ntemp), [0.1e6])
+    s = inp.System(model, n, 1, 0, 0, 2.0, 9.0, 8.0, 100, ntemp, 1, T, P)
+    s.states = inp._states(rng, mt, inp.place_lattice(n, L), T, (L, L, L), 0.5, 0.05)
+    return s
+
+
+@pytest.mark.parametrize("path", PATHS)
 @pytest.mark.parametrize("prec,tol_f,tol_e", [(0, 1e-5, 1e-5), (1, 1e-11, 1e-10)])
-def test_step0_forces_reference_fixture(golden, prec, tol_f, tol_e):
+def test_step0_forces_reference_fixture(golden, prec, tol_f, tol_e, path):
     d = golden("ref_tube_chk.npz")
     sysm = fixture_system(d, "f64")
-    g = ra.REMDGPU(sysm, precision=prec)
+    g = ra.REMDGPU(sysm, precision=prec, path=path)
+    assert g.launch_path()[0] == path
     s = g.state(0)
     ref = d["f64_init_state"]
     ids_ref = d["f64_init_ids"]
@@ -87,13 +102,20 @@ def test_step0_forces_reference_fixture(golden, prec, tol_f, tol_e):
     g.close()
 
 
+@pytest.mark.parametrize("path", PATHS)
 @pytest.mark.parametrize("model,conf", [("TIP3P", 1), ("TIP4P", 1), ("TIP5P", 1), ("TIP4P", 0), ("TIP5P", 2),
-                                        ("TIP3P", 2), ("TIP4P", 3)])
+                                        ("TIP3P", 2), ("TIP4P", 3), ("TIP5P", 4)])
 @pytest.mark.parametrize("prec", [0, 1])
-def test_step0_forces_vs_oracle(model, conf, prec):
+def test_step0_forces_vs_oracle(model, conf, prec, path):
     """conf 3: 64 molecules on two lattice planes of a bulk box, so every
-    window is the whole ring (the whole-ring window rule)."""
-    if conf == 3:
+    window is the whole ring (the whole-ring window rule); conf 4: 600
+    molecules, too large for the resident path (run on the global path)."""
+    if conf == 4:
+        if path == ra.PATH_RESIDENT:
+            pytest.skip("600 TIP5P do not fit the resident path's LDS")
+        sysm = bulk_large(model=model)
+        conf = 0
+    elif conf == 3:
         sysm = inp.config_bulk(seed=11, nmol=64, ntemp=2, model=model)
         conf = 0
     elif conf == 0:
@@ -102,7 +124,7 @@ def test_step0_forces_vs_oracle(model, conf, prec):
         sysm = inp.config_tube(seed=12, nmol=120, ntemp=2, npress=1, model=model)
     else:
         sysm = inp.config_slit(seed=13, nmol=200, ntemp=2, npress=1, model=model, Lxy=30.0)
-    g = ra.REMDGPU(sysm, precision=prec)
+    g = ra.REMDGPU(sysm, precision=prec, path=path)
     g.set_debug(True)
     g.compute_forces(do_sort=True, margin=0.0)
     tol = 1e-5 if prec == 0 else 1e-11
@@ -129,11 +151,12 @@ def test_step0_forces_vs_oracle(model, conf, prec):
 
 
 # -------------------------------------------------------------- d1 (+a-c)
+@pytest.mark.parametrize("path", PATHS)
 @pytest.mark.parametrize("prec,tol", [(1, 1e-7), (0, 1e-3)])
-def test_interval_reference_fixture(golden, prec, tol):
+def test_interval_reference_fixture(golden, prec, tol, path):
     d = golden("ref_tube_chk.npz")
     sysm = fixture_system(d, "f64")
-    g = ra.REMDGPU(sysm, precision=prec)
+    g = ra.REMDGPU(sysm, precision=prec, path=path)
     g.execute_steps(sysm.T, sysm.P)
     s = g.state(0)
     ref = d["f64_int_state"]
@@ -176,24 +199,105 @@ def test_exchange_device_matches_oracle():
     g.close()
 
 
+# ------------------------------------------------------------- paths
+@pytest.mark.parametrize("prec,tol", [(1, 1e-11), (0, 1e-5)])
+def test_resident_path_agrees_with_global_path(prec, tol):
+    """The two paths reduce in different orders; 20 steps of the 64-replica
+    bench tube (config seed 2, 4 replicas) agree to the stated tolerance."""
+    s = inp.config_tube(seed=2, ntemp=2, npress=2)
+    s.interval = 20
+    out = []
+    for path in PATHS:
+        g = ra.REMDGPU(s, precision=prec, path=path)
+        g.execute_steps(s.T, s.P)
+        out.append((g.all_averages(), [g.state(k) for k in range(4)]))
+        g.close()
+    (a0, s0), (a1, s1) = out
+    for k in range(4):
+        assert np.array_equal(s0[k]["id"], s1[k]["id"])
+        dd = s0[k]["r"] - s1[k]["r"]
+        dd -= np.rint(dd)
+        assert np.max(np.abs(dd)) <= tol
+        assert rel_rms(s0[k]["p"], s1[k]["p"]) <= 1e3 * tol
+    assert np.allclose(a0[:, :3], a1[:, :3], rtol=1e3 * tol, atol=1e-6)
+
+
+def test_parts_do_not_change_results():
+    """A replica split over 7 workgroups (alone) and over 1 (batched with a
+    254-replica filler job that takes every slot) gives the same bits."""
+    a = inp.config_tube(seed=31, nmol=60, Lz=26.0, ntemp=2, npress=1, nfwall=500)
+    a.interval = 20
+    g = ra.REMDGPU(a)
+    g.execute_steps(a.T, a.P)
+    parts_alone = g.launch_path()[1]
+    ref = (g.all_averages(), g.state(0)["r"], g.state(1)["p"])
+    g.close()
+    filler = inp.config_tube(seed=33, nmol=16, Lz=22.0, ntemp=127, npress=2, nfwall=300)
+    filler.interval = 20
+    ga, gf = ra.REMDGPU(a), ra.REMDGPU(filler)
+    bt = ra.Batch()
+    bt.add(ga)
+    bt.add(gf)
+    ga.set_conditions(a.T, a.P)
+    gf.set_conditions(filler.T, filler.P)
+    bt.execute()
+    parts_batch = ga.launch_path()[1]
+    print("parts alone", parts_alone, "batched", parts_batch)
+    assert parts_alone > 1 and parts_batch == 1
+    assert np.array_equal(ga.all_averages(), ref[0])
+    assert np.array_equal(ga.state(0)["r"], ref[1])
+    assert np.array_equal(ga.state(1)["p"], ref[2])
+    bt.close()
+    ga.close()
+    gf.close()
+
+
 # ------------------------------------------------------------- batching
-def test_batch_equals_alone_and_plain():
+def test_mixed_path_batch_equals_alone():
+    """A resident job and a global-state job batched together (two kernel
+    classes in one interval) give the bits each gives alone."""
+    a = inp.config_tube(seed=31, nmol=60, Lz=26.0, ntemp=2, npress=1, nfwall=500)
+    b = inp.config_tube(seed=32, nmol=48, Lz=24.0, ntemp=2, npress=1, model="TIP3P", nfwall=500)
+    for s in (a, b):
+        s.interval = 20
+    alone = []
+    for s, path in ((a, ra.PATH_RESIDENT), (b, ra.PATH_GLOBAL)):
+        g = ra.REMDGPU(s, path=path)
+        g.execute_steps(s.T, s.P)
+        alone.append(g.all_averages())
+        g.close()
+    ga, gb = ra.REMDGPU(a, path=ra.PATH_RESIDENT), ra.REMDGPU(b, path=ra.PATH_GLOBAL)
+    bt = ra.Batch()
+    bt.add(ga)
+    bt.add(gb)
+    ga.set_conditions(a.T, a.P)
+    gb.set_conditions(b.T, b.P)
+    bt.execute()
+    assert np.array_equal(ga.all_averages(), alone[0])
+    assert np.array_equal(gb.all_averages(), alone[1])
+    bt.close()
+    ga.close()
+    gb.close()
+
+
+@pytest.mark.parametrize("path", PATHS)
+def test_batch_equals_alone_and_plain(path):
     a = inp.config_tube(seed=31, nmol=60, Lz=26.0, ntemp=2, npress=1, nfwall=500)
     b = inp.config_tube(seed=32, nmol=48, Lz=24.0, ntemp=2, npress=1, model="TIP3P", nfwall=500)
     for s in (a, b):
         s.interval = 20
     alone = {}
     for name, s in (("a", a), ("b", b)):
-        g = ra.REMDGPU(s)
+        g = ra.REMDGPU(s, path=path)
         g.execute_steps(s.T, s.P)
         alone[name] = (g.all_averages(), g.state(0)["r"])
         g.close()
     # plain (one replica per launch) gives the same bits
-    g = ra.REMDGPU(a)
+    g = ra.REMDGPU(a, path=path)
     g.execute_steps(a.T, a.P, plain=True)
     assert np.array_equal(g.all_averages(), alone["a"][0])
     g.close()
-    ga, gb = ra.REMDGPU(a), ra.REMDGPU(b)
+    ga, gb = ra.REMDGPU(a, path=path), ra.REMDGPU(b, path=path)
     bt = ra.Batch()
     bt.add(ga)
     bt.add(gb)
