@@ -133,8 +133,8 @@ int remd_gpu_get_window(remd_gpu *g, int rep, int *js, int *je, int *full);
  * remd_gpu_execute_steps call, from HIP events on the job's stream. */
 int remd_gpu_last_kernel_ms(remd_gpu *g, double *ms);
 /* Phase timers of the last interval (REMD_PHASE_TIMERS builds; zeros
- * otherwise): out[8] in 100 MHz ticks accumulated over the job so far. */
-int remd_gpu_phase_timers(remd_gpu *g, int rep, double out[8]);
+ * otherwise): out[16] in 100 MHz ticks accumulated over the job so far. */
+int remd_gpu_phase_timers(remd_gpu *g, int rep, double out[16]);
 /* Chosen launch shape. */
 int remd_gpu_launch_shape(remd_gpu *g, int *block_threads, int *chunks, int *lds_bytes);
 /* Path of the job and the workgroups per replica its last launch used. */

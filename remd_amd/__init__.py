@@ -273,7 +273,7 @@ class REMDGPU:
         return ms.value
 
     def phase_timers(self, rep=0):
-        out = np.zeros(8)
+        out = np.zeros(16)
         _check(lib().remd_gpu_phase_timers(self.h, int(rep), _dp_of(out)))
         return out
 

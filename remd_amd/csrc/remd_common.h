@@ -43,8 +43,8 @@ enum Scalar {
   S_PRS0, S_PRS1, S_PRS2, S_PRS3,
   S_VOL,
   S_STEPS,             // steps integrated so far (diagnostic)
-  S_DIAG0,             // 8 phase timers (REMD_PHASE_TIMERS builds only)
-  NSC = S_DIAG0 + 8
+  S_DIAG0,             // 16 phase timers (REMD_PHASE_TIMERS builds only)
+  NSC = S_DIAG0 + 16
 };
 
 // kernel modes
@@ -99,8 +99,12 @@ struct LaunchArgs {
   int do_sort;                  // MODE_FORCE only
   double margin;                // MODE_FORCE only
   int parts;                    // resident path: workgroups per replica
-  unsigned *sync;               // resident path, parts > 1: nrep arrival counters, zeroed
+  unsigned long long *sync;     // resident path, parts > 1: nrep group words, zeroed (low 32
+                                // bits: arrivals; high 32: XCD histogram of the parts)
   unsigned *abort;              // resident path, parts > 1: abort word, zeroed
+  int same_xcd_ok;              // resident path: may use the same-XCD group barrier
+  int same_xcd_mode;            // same-XCD barrier: bit 0 agent-scope release, bit 1 agent-scope
+                                // acquire (else: stores drained to L2 / device-scope loads)
 };
 
 }  // namespace remd

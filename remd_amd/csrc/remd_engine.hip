@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -107,7 +108,7 @@ struct remd_gpu {
   bool resident = false;     // state in LDS for the launch (resident_kernel)
   int capacity = 0;          // resident workgroups the device holds at once
   int last_parts = 1;
-  unsigned *d_sync = nullptr;  // nrep arrival counters + abort word
+  unsigned long long *d_sync = nullptr;  // nrep group words + the abort word
   double *d_xbuf = nullptr;    // nrep x 2 x N x XW exchange rows
   unsigned *h_abort = nullptr; // pinned copy of the abort word
   hipStream_t stream = nullptr;
@@ -150,7 +151,7 @@ struct remd_batch {
   int nt = 256;
   int cls_off[NCLASS] = {0}, cls_n[NCLASS] = {0}, cls_lds[NCLASS] = {0};
   int res_capacity = 0, res_minN = 0, res_reps = 0, parts = 1;
-  unsigned *d_sync = nullptr;  // ntab arrival counters + one abort word
+  unsigned long long *d_sync = nullptr;  // ntab group words + the abort word
   unsigned *h_abort = nullptr;
   hipStream_t stream = nullptr;
   hipStream_t cstream[NCLASS] = {nullptr};
@@ -346,6 +347,26 @@ static int upload_reps(remd_gpu *g) {
   return 0;
 }
 
+// The same-XCD group barrier (L1-only invalidation) may be switched off
+// with REMD_SAME_XCD_BARRIER=0; the parts then always use agent-scope fences.
+static int same_xcd_barrier_enabled() {
+  static int v = -1;
+  if (v < 0) {
+    const char *e = std::getenv("REMD_SAME_XCD_BARRIER");
+    v = (e && e[0] == '0') ? 0 : 1;
+  }
+  return v;
+}
+
+static int same_xcd_barrier_mode() {
+  static int v = -1;
+  if (v < 0) {
+    const char *e = std::getenv("REMD_BARRIER_MODE");
+    v = e ? (std::atoi(e) & 3) : 0;
+  }
+  return v;
+}
+
 // Workgroups per replica for a resident launch of nrep replicas that has
 // `capacity` co-resident workgroup slots to itself: every part of every
 // replica must be resident at once (the parts wait for each other), and a
@@ -367,11 +388,13 @@ static int launch(remd_gpu *g, int mode, int do_sort, double margin, const Repli
   a.do_sort = do_sort;
   a.margin = margin;
   a.parts = 1;
+  a.same_xcd_ok = same_xcd_barrier_enabled();
+  a.same_xcd_mode = same_xcd_barrier_mode();
   a.sync = g->d_sync;
-  a.abort = g->d_sync ? g->d_sync + g->nrep : nullptr;
+  a.abort = g->d_sync ? (unsigned *)(g->d_sync + g->nrep) : nullptr;
   if (g->resident) {
     a.parts = pick_parts(g->capacity, nrep, g->N);
-    if (a.parts > 1) HIPCHK(hipMemsetAsync(g->d_sync, 0, sizeof(unsigned) * (nrep + 1), g->stream));
+    if (a.parts > 1) HIPCHK(hipMemsetAsync(g->d_sync, 0, sizeof(unsigned long long) * (g->nrep + 1), g->stream));
   }
   g->last_parts = a.parts;
   if (timed) HIPCHK(hipEventRecord(g->ev0, g->stream));
@@ -478,8 +501,8 @@ extern "C" int remd_gpu_create(remd_gpu **out, const remd_gpu_desc *d, const rem
     HIPCHK(hipMalloc(&g->d_reps, sizeof(ReplicaDesc) * R));
     HIPCHK(hipHostMalloc(&g->h_sc, sizeof(double) * NSC * R, hipHostMallocDefault));
     if (g->resident) {
-      HIPCHK(hipMalloc(&g->d_sync, sizeof(unsigned) * (R + 1)));
-      HIPCHK(hipMemsetAsync(g->d_sync, 0, sizeof(unsigned) * (R + 1), g->stream));
+      HIPCHK(hipMalloc(&g->d_sync, sizeof(unsigned long long) * (R + 1)));
+      HIPCHK(hipMemsetAsync(g->d_sync, 0, sizeof(unsigned long long) * (R + 1), g->stream));
       HIPCHK(hipMalloc(&g->d_xbuf, sizeof(double) * R * 2 * N * XW));
       HIPCHK(hipHostMalloc(&g->h_abort, sizeof(unsigned), hipHostMallocDefault));
       *g->h_abort = 0u;
@@ -587,7 +610,7 @@ extern "C" int remd_gpu_execute_steps_plain(remd_gpu *g, const double *T, const 
   // takes as many workgroups as it can use
   const int parts = g->resident ? pick_parts(g->capacity, 1, g->N) : 1;
   g->last_parts = parts;
-  if (parts > 1) HIPCHK(hipMemsetAsync(g->d_sync, 0, sizeof(unsigned) * (g->nrep + 1), g->stream));
+  if (parts > 1) HIPCHK(hipMemsetAsync(g->d_sync, 0, sizeof(unsigned long long) * (g->nrep + 1), g->stream));
   HIPCHK(hipEventRecord(g->ev0, g->stream));
   for (int r = 0; r < g->nrep; r++) {
     LaunchArgs a;
@@ -597,8 +620,10 @@ extern "C" int remd_gpu_execute_steps_plain(remd_gpu *g, const double *T, const 
     a.do_sort = 0;
     a.margin = 0.0;
     a.parts = parts;
+    a.same_xcd_ok = same_xcd_barrier_enabled();
+    a.same_xcd_mode = same_xcd_barrier_mode();
     a.sync = g->d_sync ? g->d_sync + r : nullptr;
-    a.abort = g->d_sync ? g->d_sync + g->nrep : nullptr;
+    a.abort = g->d_sync ? (unsigned *)(g->d_sync + g->nrep) : nullptr;
     if (g->resident) HIPCHK(launch_resident(a, g->NS, g->prm.precision, g->lds, g->stream));
     else HIPCHK(launch_interval(a, g->nt, g->NS, g->prm.precision, g->lds, g->stream));
   }
@@ -668,9 +693,9 @@ extern "C" int remd_gpu_last_kernel_ms(remd_gpu *g, double *ms) {
   return 0;
 }
 
-extern "C" int remd_gpu_phase_timers(remd_gpu *g, int rep, double out[8]) {
+extern "C" int remd_gpu_phase_timers(remd_gpu *g, int rep, double out[16]) {
   if (!g || !out || rep < 0 || rep >= g->nrep) return fail(REMD_EINVAL, "bad argument");
-  for (int k = 0; k < 8; k++) out[k] = g->h_sc[(size_t)rep * NSC + S_DIAG0 + k];
+  for (int k = 0; k < 16; k++) out[k] = g->h_sc[(size_t)rep * NSC + S_DIAG0 + k];
   return 0;
 }
 
@@ -1078,7 +1103,7 @@ static int rebuild_table(remd_batch *b) {
   }
   hipFree(b->d_sync);
   b->d_sync = nullptr;
-  HIPCHK(hipMalloc(&b->d_sync, sizeof(unsigned) * (n + 1)));
+  HIPCHK(hipMalloc(&b->d_sync, sizeof(unsigned long long) * (n + 1)));
   if (!b->h_abort) HIPCHK(hipHostMalloc(&b->h_abort, sizeof(unsigned), hipHostMallocDefault));
   *b->h_abort = 0u;
   std::vector<ReplicaDesc> flat(n);
@@ -1101,7 +1126,7 @@ extern "C" int remd_batch_execute(remd_batch *b) {
   }
   // conditions were staged on each job's stream; order them before the launch
   for (remd_gpu *g : b->jobs) HIPCHK(hipStreamSynchronize(g->stream));
-  if (b->parts > 1) HIPCHK(hipMemsetAsync(b->d_sync, 0, sizeof(unsigned) * (b->ntab + 1), b->stream));
+  if (b->parts > 1) HIPCHK(hipMemsetAsync(b->d_sync, 0, sizeof(unsigned long long) * (b->ntab + 1), b->stream));
   HIPCHK(hipEventRecord(b->ev0, b->stream));
   // resident classes first (concurrently), then the global-state classes
   for (int pass = 0; pass < 2; pass++) {
@@ -1118,8 +1143,10 @@ extern "C" int remd_batch_execute(remd_batch *b) {
       a.do_sort = 0;
       a.margin = 0.0;
       a.parts = res ? b->parts : 1;
+      a.same_xcd_ok = same_xcd_barrier_enabled();
+      a.same_xcd_mode = same_xcd_barrier_mode();
       a.sync = b->d_sync + b->cls_off[c];
-      a.abort = b->d_sync + b->ntab;
+      a.abort = (unsigned *)(b->d_sync + b->ntab);
       const int nsite = 3 + (c % 6) / 2, prec = c % 2;
       if (res) HIPCHK(launch_resident(a, nsite, prec, b->cls_lds[c], b->cstream[c]));
       else HIPCHK(launch_interval(a, b->nt, nsite, prec, b->cls_lds[c], b->cstream[c]));

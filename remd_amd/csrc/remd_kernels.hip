@@ -52,13 +52,14 @@ typedef const REMD_AS_CONST JobParams &JRef;
 
 
 // Optional phase timers (constant 100 MHz clock), accumulated by thread 0
-// into S_DIAG0.. of the replica's scalar block; compiled out by default.
+// into 16 accumulators (the S_DIAG0.. words of the replica's scalar block,
+// or LDS on the resident path); compiled out by default.
 #ifdef REMD_PHASE_TIMERS
 #define PT_DECL uint64_t pt_t0 = wall_clock64();
 #define PT_MARK(k, sc)                                        \
   do {                                                        \
     const uint64_t pt_t1 = wall_clock64();                    \
-    if (threadIdx.x == 0 && (sc)) (sc)[S_DIAG0 + (k)] += (double)(pt_t1 - pt_t0); \
+    if (threadIdx.x == 0 && (sc)) (sc)[(k)] += (double)(pt_t1 - pt_t0); \
     pt_t0 = pt_t1;                                            \
   } while (0)
 #else
@@ -1091,14 +1092,14 @@ __device__ void md_step(JRef J, double *mol, int *mid, int N, Sc &c, const Lds &
                         bool do_sort, const DescD &R) {
   PT_DECL
   first_half<NT>(J, mol, N, c, S.red);
-  PT_MARK(0, gen(R.sc));
+  PT_MARK(0, gen(R.sc) + S_DIAG0);
   // force
   if (do_sort) sort_and_window<NT>(J, mol, mid, N, J.sort_axis == 0 ? c.L[0] : c.L[2], J.margin, S, gen(R.win));
-  PT_MARK(1, gen(R.sc));
-  force_pass<NT, NS, PT>(J, mol, N, c, S, gen(R.site_f), gen(R.sc));  // ends with block_sum (synced)
-  PT_MARK(2, gen(R.sc));
+  PT_MARK(1, gen(R.sc) + S_DIAG0);
+  force_pass<NT, NS, PT>(J, mol, N, c, S, gen(R.site_f), gen(R.sc) + S_DIAG0);  // ends with block_sum (synced)
+  PT_MARK(2, gen(R.sc) + S_DIAG0);
   second_half<NT>(J, mol, N, c);
-  PT_MARK(3, gen(R.sc));
+  PT_MARK(3, gen(R.sc) + S_DIAG0);
 }
 
 // --------------------------------------------------------- replica driver
@@ -1115,7 +1116,7 @@ __device__ void run_replica(const LaunchArgs &a, const DescD &R, JRef J, char *s
     // InitializeProperty (src/molecules.h:260-270): sort with margin 0,
     // force, gkT, CalcProperties, H0 = tot.
     sort_and_window<NT>(J, mol, mid, N, J.sort_axis == 0 ? c.L[0] : c.L[2], 0.0, S, gen(R.win));
-    force_pass<NT, NS, PT>(J, mol, N, c, S, gen(R.site_f), gen(R.sc));
+    force_pass<NT, NS, PT>(J, mol, N, c, S, gen(R.site_f), gen(R.sc) + S_DIAG0);
     c.gkT = 6.0 * (double)N * c.T;
     const double tot = total_energy<NT>(J, mol, N, c, S.red);
     c.H0 = tot;
@@ -1141,7 +1142,7 @@ __device__ void run_replica(const LaunchArgs &a, const DescD &R, JRef J, char *s
       }
       __syncthreads();
     }
-    force_pass<NT, NS, PT>(J, mol, N, c, S, gen(R.site_f), gen(R.sc));
+    force_pass<NT, NS, PT>(J, mol, N, c, S, gen(R.site_f), gen(R.sc) + S_DIAG0);
     calc_properties<NT>(J, mol, N, c, gen(R.sc), S.red);
     if (threadIdx.x == 0) store_sc(c, gen(R.sc));
     return;
@@ -1222,7 +1223,8 @@ struct LdsR {
   double *part;     // NV x NT pair partials, value-major
   double *E;        // 6 x N per-molecule lj, clmb, wall, vir[3]
   double *red;      // block_sum scratch
-  int *flag;        // barrier status broadcast
+  int *flag;        // barrier status broadcast (16 ints)
+  double *ptimer;   // 16 phase-timer accumulators (REMD_PHASE_TIMERS builds)
 };
 
 __host__ __device__ inline int resident_region0(int nmol, int nsite, int prec) {
@@ -1235,7 +1237,7 @@ __host__ __device__ inline int resident_region0(int nmol, int nsite, int prec) {
 // Bytes of dynamic LDS the resident path needs for one replica.
 __host__ __device__ inline int resident_lds_bytes(int nmol, int nsite, int prec, int nt) {
   return align16(NFIELD * 8 * nmol) + align16(4 * nmol) + resident_region0(nmol, nsite, prec) +
-         align16(4 * nmol) + (3 * nsite + 8) * 8 * nt + align16(6 * 8 * nmol) + (nt / WAVE) * 32 * 8 + 64;
+         align16(4 * nmol) + (3 * nsite + 8) * 8 * nt + align16(6 * 8 * nmol) + (nt / WAVE) * 32 * 8 + 64 + 128;
 }
 
 template <int NT, int NS>
@@ -1258,32 +1260,57 @@ __device__ inline LdsR carve_resident(char *smem, int N, int prec) {
   L.red = (double *)p;
   p += (NT / WAVE) * 32 * 8;
   L.flag = (int *)p;
+  p += 64;
+  L.ptimer = (double *)p;
   return L;
 }
 
 struct Group {
   int parts, part;
-  unsigned *ctr;    // arrivals of this replica's parts, zero at launch start
+  unsigned long long *ctr;  // low 32 bits: arrivals of this replica's parts; high 32 bits: 4-bit
+                            // count of parts per XCD (added at the first barrier); zero at start
   unsigned *abort;  // nonzero once any wait of the launch timed out
   double *xbuf;     // 2 x N x XW exchange rows (double-buffered by step parity)
   unsigned epoch;   // group barriers completed in this launch
+  bool same_xcd;    // every part of the replica shares one L2
+  int mode;         // same-XCD barrier variant (LaunchArgs::same_xcd_mode)
 };
 
+// XCD (XCC) this workgroup runs on.
+__device__ inline unsigned xcc_id() { return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 15u; }
+
 // Barrier across the parts of one replica.  Every thread first makes its
-// own global stores visible at device scope; one thread counts in and polls
-// the counter (bounded; a timeout raises the launch-wide abort flag so every
-// part leaves); every thread then acquires before reading what the other
-// parts wrote.  Returns false when the launch is aborting.
+// own global stores visible to the other parts; one thread counts in and
+// polls the counter (bounded; a timeout raises the launch-wide abort flag so
+// every part leaves); then the other parts' stores are made visible to this
+// part's loads.  When all parts share one XCD (and so one L2, the coherence
+// point of its CUs) the stores are only drained to L2 and the exchange rows
+// are read with device-scope loads that bypass the CU's L1 (an L1-only
+// invalidate was measured not to suffice); otherwise agent-scope release and
+// acquire fences (L2 write-back / invalidate) and ordinary loads.  Returns
+// false when the launch is aborting.
+// `census` (the launch's first barrier): each part also adds 1 to the
+// 4-bit field of its XCD in the high half of the group word, and flag[1]
+// receives whether all parts share one XCD -- read from the same word that
+// completes the barrier, so every part sees every part's field and all
+// parts decide alike.
 template <int NT>
-__device__ bool group_wait(Group &g, int *flag) {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+__device__ bool group_wait(Group &g, int *flag, double *pt_sc = nullptr, bool census = false) {
+  PT_DECL
+  (void)pt_sc;
+  if (g.same_xcd && !(g.mode & 1)) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  else __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   __syncthreads();
+  PT_MARK(8, pt_sc);
   if (threadIdx.x == 0) {
     int ok = 1;
-    const unsigned target = (unsigned)g.parts * (g.epoch + 1u);
-    __hip_atomic_fetch_add(g.ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long target = (unsigned long long)g.parts * (g.epoch + 1u);
+    const unsigned long long inc = 1ull + (census ? (1ull << (32 + 4 * xcc_id())) : 0ull);
+    __hip_atomic_fetch_add(g.ctr, inc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     unsigned spins = 0;
-    while (__hip_atomic_load(g.ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    unsigned long long v;
+    while (((v = __hip_atomic_load(g.ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) & 0xffffffffull) <
+           target) {
       ++spins;
       if ((spins & 63u) == 0u) {
         if (__hip_atomic_load(g.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
@@ -1299,12 +1326,20 @@ __device__ bool group_wait(Group &g, int *flag) {
       __builtin_amdgcn_s_sleep(2);
     }
     *flag = ok;
+    if (census) {
+      const unsigned hist = (unsigned)(v >> 32);
+      int nz = 0;
+      for (int x = 0; x < 8; x++) nz += ((hist >> (4 * x)) & 15u) != 0u;
+      flag[1] = (nz == 1) ? 1 : 0;
+    }
   }
   __syncthreads();
+  PT_MARK(9, pt_sc);
   const bool ok = *flag != 0;
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  if (!g.same_xcd || (g.mode & 2)) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   g.epoch++;
   __syncthreads();  // flag is rewritten by the next barrier
+  PT_MARK(10, pt_sc);
   return ok;
 }
 
@@ -1601,21 +1636,31 @@ __device__ bool forces_resident(JRef J, double *mol, int N, Sc &c, const LdsR &S
     double *xrow = g.xbuf + (size_t)(g.epoch & 1u) * N * XW;
     force_range<NT, NS, PT>(J, mol, N, c, S, i0, i1, xrow, site_dbg);
     PT_MARK(5, pt_sc);
-    if (!group_wait<NT>(g, S.flag)) return false;
+    if (!group_wait<NT>(g, S.flag, pt_sc)) return false;
     PT_MARK(6, pt_sc);
+    const bool direct = g.same_xcd && !(g.mode & 2);
     for (int i = threadIdx.x; i < N; i += NT) {
-      const double *o = xrow + (size_t)i * XW;
-      mol[F_FX * N + i] = o[0];
-      mol[F_FY * N + i] = o[1];
-      mol[F_FZ * N + i] = o[2];
-      mol[F_N0 * N + i] = o[3];
-      mol[F_N1 * N + i] = o[4];
-      mol[F_N2 * N + i] = o[5];
-      mol[F_N3 * N + i] = o[6];
+      double *o = xrow + (size_t)i * XW;
+      double v[13];
+      if (direct) {
 #pragma unroll
-      for (int u = 0; u < 6; u++) S.E[u * N + i] = o[7 + u];
+        for (int u = 0; u < 13; u++) v[u] = __hip_atomic_load(o + u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+#pragma unroll
+        for (int u = 0; u < 13; u++) v[u] = o[u];
+      }
+      mol[F_FX * N + i] = v[0];
+      mol[F_FY * N + i] = v[1];
+      mol[F_FZ * N + i] = v[2];
+      mol[F_N0 * N + i] = v[3];
+      mol[F_N1 * N + i] = v[4];
+      mol[F_N2 * N + i] = v[5];
+      mol[F_N3 * N + i] = v[6];
+#pragma unroll
+      for (int u = 0; u < 6; u++) S.E[u * N + i] = v[7 + u];
     }
     __syncthreads();
+    PT_MARK(11, pt_sc);
   } else {
     force_range<NT, NS, PT>(J, mol, N, c, S, 0, N, nullptr, site_dbg);
     PT_MARK(5, pt_sc);
@@ -1651,8 +1696,17 @@ __device__ void run_resident(const LaunchArgs &a, const DescD &R, JRef J, char *
   const double xflag = gen(R.sc)[S_XFLAG];
   const double Tn = gen(R.sc)[S_TNEW];
   const double Pn = gen(R.sc)[S_PNEW];
+  if (threadIdx.x < 16) S.ptimer[threadIdx.x] = 0.0;
   __syncthreads();
   const bool lead = g.part == 0;
+  if (g.parts > 1) {
+    // one fully fenced barrier first; it also tells whether every part of
+    // the replica runs on one XCD
+    g.same_xcd = false;
+    if (!group_wait<NT>(g, S.flag, nullptr, true)) return;
+    g.same_xcd = a.same_xcd_ok && S.flag[1] != 0;
+    __syncthreads();
+  }
   const int i0 = (int)(((long long)g.part * N) / g.parts);
   const int i1 = (int)(((long long)(g.part + 1) * N) / g.parts);
   double *site_dbg = gen(R.site_f);
@@ -1693,7 +1747,7 @@ __device__ void run_resident(const LaunchArgs &a, const DescD &R, JRef J, char *
       c.H0 = total_energy<NT>(J, mol, N, c, S.red);
     }
     if (J.tmode == 2) velocity_scaling<NT>(J, mol, N, c, S.red);
-    double *pt = lead ? gen(R.sc) : nullptr;
+    double *pt = lead ? S.ptimer : nullptr;
     (void)pt;
     for (int st = 0; st < J.interval; ++st) {
       PT_DECL
@@ -1716,6 +1770,11 @@ __device__ void run_resident(const LaunchArgs &a, const DescD &R, JRef J, char *
     for (int k = threadIdx.x; k < NFIELD * N; k += NT) gen(R.mol)[k] = mol[k];
     for (int i = threadIdx.x; i < N; i += NT) gen(R.mid)[i] = mid[i];
     if (threadIdx.x == 0) {
+#ifdef REMD_PHASE_TIMERS
+      for (int k = 0; k < 14; k++) gen(R.sc)[S_DIAG0 + k] += S.ptimer[k];
+      gen(R.sc)[S_DIAG0 + 14] = (double)xcc_id();
+      gen(R.sc)[S_DIAG0 + 15] = g.same_xcd ? 1.0 : 0.0;
+#endif
       store_sc(c, gen(R.sc));
       if (a.mode == MODE_INTERVAL) {
         gen(R.sc)[S_XFLAG] = 0.0;
@@ -1750,6 +1809,8 @@ __global__ __launch_bounds__(NT) void resident_kernel(LaunchArgs a) {
   g.part = part;
   g.ctr = a.sync + rep;
   g.abort = a.abort;
+  g.same_xcd = false;
+  g.mode = a.same_xcd_mode;
   g.xbuf = gen(R.xbuf);
   g.epoch = 0;
   run_resident<NT, NS, PT>(a, R, J, smem, g);
