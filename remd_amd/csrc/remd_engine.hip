@@ -22,7 +22,7 @@ hipError_t launch_interval(const LaunchArgs &a, int nt, int nsite, int prec, int
 hipError_t set_kernel_lds_limit(int bytes);
 hipError_t launch_resident(const LaunchArgs &a, int nsite, int prec, int lds, hipStream_t st);
 hipError_t resident_occupancy(int nsite, int prec, int lds, int *per_cu);
-int resident_lds(int nmol, int nsite, int prec);
+int resident_lds(int nmol, int nsite, int prec, int site_dbg);
 hipError_t launch_wall_tube(int nfwall, double R, double sgm06, double coefE, double *fw, hipStream_t st);
 hipError_t launch_wall_slit(int nfwall, double H, double sgm, double eps, double rho, double *fw,
                             hipStream_t st);
@@ -441,7 +441,7 @@ extern "C" int remd_gpu_create(remd_gpu **out, const remd_gpu_desc *d, const rem
   const int lim = device_lds_limit();
   // window fraction along the sort axis ~ 2 (rcut + margin) / L
   const double wf = std::min(1.0, 2.0 * (d->rcut + d->margin) / sc[0].L[g->prm.sort_axis]);
-  const int rlds = resident_lds(g->N, g->NS, g->prm.precision);
+  const int rlds = resident_lds(g->N, g->NS, g->prm.precision, 0);
   const bool fits = g->nt == 256 && rlds <= lim;
   if (d->path == REMD_PATH_RESIDENT && !fits) {
     free_job(g);
@@ -658,6 +658,12 @@ extern "C" int remd_gpu_set_debug(remd_gpu *g, int on) {
     HIPCHK(hipMalloc(&g->d_win, sizeof(int) * g->nrep * g->N * 3));
     HIPCHK(hipMemsetAsync(g->d_site, 0, sizeof(double) * g->nrep * g->N * g->NS * 3, g->stream));
     HIPCHK(hipMemsetAsync(g->d_win, 0, sizeof(int) * g->nrep * g->N * 3, g->stream));
+  }
+  if (g->resident) {
+    // the site-force partials take extra LDS on the resident path
+    const int lds = resident_lds(g->N, g->NS, g->prm.precision, on != 0);
+    if (lds > device_lds_limit()) return fail(REMD_EINVAL, "site-force debug buffers do not fit this replica's LDS");
+    g->lds = lds;
   }
   g->debug = on != 0;
   int rc = upload_reps(g);

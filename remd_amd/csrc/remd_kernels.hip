@@ -194,6 +194,21 @@ __device__ inline void space_to_body(const double *a, const double *v, double *t
   t[2] = 2. * (a13 + a02) * v[0] + 2. * (a23 - a01) * v[1] + (a0sq - a1sq - a2sq + a3sq) * v[2];
 }
 
+// sin and cos of a rotation sub-step angle.  Below 0.125 rad the Taylor
+// series through x^11 (sin) and x^12 (cos) leaves a truncation error under
+// 1e-21 relative, below FP64 rounding; larger angles use sincos.
+__device__ inline void step_sincos(double x, double *sn, double *cs) {
+  if (fabs(x) < 0.125) {
+    const double x2 = x * x;
+    *sn = x * (1.0 + x2 * (-1.0 / 6.0 + x2 * (1.0 / 120.0 + x2 * (-1.0 / 5040.0 + x2 * (1.0 / 362880.0 +
+                                                                   x2 * (-1.0 / 39916800.0))))));
+    *cs = 1.0 + x2 * (-0.5 + x2 * (1.0 / 24.0 + x2 * (-1.0 / 720.0 + x2 * (1.0 / 40320.0 +
+                                                                x2 * (-1.0 / 3628800.0 + x2 * (1.0 / 479001600.0))))));
+  } else {
+    sincos(x, sn, cs);
+  }
+}
+
 // Free rotation about body axis k (src/integrator.cxx:467-563, P1..P3 of
 // src/integrator.h:321-346).  Returns xi.
 __device__ inline double rotate(double *q, double *p, int axis, double coef_xi, double I, double h) {
@@ -210,7 +225,7 @@ __device__ inline double rotate(double *q, double *p, int axis, double coef_xi, 
   }
   const double xi = (p[0] * Pq[0] + p[1] * Pq[1] + p[2] * Pq[2] + p[3] * Pq[3]) * coef_xi / I;
   double sn, c;
-  sincos(xi * h, &sn, &c);
+  step_sincos(xi * h, &sn, &c);
 #pragma unroll
   for (int d = 0; d < 4; d++) {
     const double qn = q[d] * c + Pq[d] * sn;
@@ -1210,10 +1225,11 @@ __global__ __launch_bounds__(NT) void interval_kernel(LaunchArgs a) {
 constexpr int XW = 16;                 // doubles per molecule row of the force exchange
 constexpr unsigned MAXSPIN = 1u << 20; // group-barrier polls before the launch gives up
 
-template <int NS>
-struct NVal {
-  static constexpr int V = 3 * NS + 8;  // site forces, COM correction, virial, lj, clmb
-};
+// Per-item partials of the pair pass: COM force (3), quaternion torque (4),
+// virial (3), lj, clmb, wall energy -- plus the site forces (3 NS) when the
+// site-force debug buffer is on.
+constexpr int NPART = 13;
+__host__ __device__ inline int npart(int nsite, bool site_dbg) { return NPART + (site_dbg ? 3 * nsite : 0); }
 
 struct LdsR {
   double *mol;      // NFIELD x N replica state
@@ -1235,13 +1251,14 @@ __host__ __device__ inline int resident_region0(int nmol, int nsite, int prec) {
 }
 
 // Bytes of dynamic LDS the resident path needs for one replica.
-__host__ __device__ inline int resident_lds_bytes(int nmol, int nsite, int prec, int nt) {
+__host__ __device__ inline int resident_lds_bytes(int nmol, int nsite, int prec, int nt, bool site_dbg) {
   return align16(NFIELD * 8 * nmol) + align16(4 * nmol) + resident_region0(nmol, nsite, prec) +
-         align16(4 * nmol) + (3 * nsite + 8) * 8 * nt + align16(6 * 8 * nmol) + (nt / WAVE) * 32 * 8 + 64 + 128;
+         align16(4 * nmol) + npart(nsite, site_dbg) * 8 * nt + align16(6 * 8 * nmol) + (nt / WAVE) * 32 * 8 + 64 +
+         128;
 }
 
 template <int NT, int NS>
-__device__ inline LdsR carve_resident(char *smem, int N, int prec) {
+__device__ inline LdsR carve_resident(char *smem, int N, int prec, bool site_dbg) {
   LdsR L;
   char *p = smem;
   L.mol = (double *)p;
@@ -1254,7 +1271,7 @@ __device__ inline LdsR carve_resident(char *smem, int N, int prec) {
   L.je = L.js + N;
   p += align16(4 * N);
   L.part = (double *)p;
-  p += NVal<NS>::V * 8 * NT;
+  p += npart(NS, site_dbg) * 8 * NT;
   L.E = (double *)p;
   p += align16(6 * 8 * N);
   L.red = (double *)p;
@@ -1347,7 +1364,7 @@ __device__ bool group_wait(Group &g, int *flag, double *pt_sc = nullptr, bool ce
 // stable sort on the double keys: rank = #smaller + #equal-and-earlier),
 // permutation of the state, then the windows (src/molecules.cxx:541-585
 // semantics; raw js/je and the whole-ring flag go to win_dbg when given).
-template <int NT>
+template <int NT, int NS>
 __device__ void rank_sort_resident(JRef J, double *mol, int *mid, int N, double Laxis,
                                    double margin, const LdsR &S, int *win_dbg) {
   double *z = (double *)S.region0;
@@ -1366,11 +1383,17 @@ __device__ void rank_sort_resident(JRef J, double *mol, int *mid, int N, double 
     rk[i] = (uint16_t)r;
   }
   __syncthreads();
-  for (int f = 0; f < NFIELD; ++f) {
-    double *col = mol + f * N;
-    for (int i = threadIdx.x; i < N; i += NT) tmp[rk[i]] = col[i];
+  // permute the state through region 0, as many fields per round as fit
+  const int fit = (resident_region0(N, NS, J.precision) - align16(10 * N)) / (8 * N);
+  const int per = fit < 1 ? 1 : (fit > NFIELD ? NFIELD : fit);
+  for (int f0 = 0; f0 < NFIELD; f0 += per) {
+    const int nf = (f0 + per <= NFIELD) ? per : NFIELD - f0;
+    for (int k = threadIdx.x; k < nf * N; k += NT) {
+      const int f = k / N, i = k - f * N;
+      tmp[f * N + rk[i]] = mol[(f0 + f) * N + i];
+    }
     __syncthreads();
-    for (int i = threadIdx.x; i < N; i += NT) col[i] = tmp[i];
+    for (int k = threadIdx.x; k < nf * N; k += NT) mol[f0 * N + k] = tmp[k];
     __syncthreads();
   }
   {
@@ -1428,10 +1451,11 @@ __device__ void window_from_debug(const int *win, int N, const LdsR &S) {
 // LDS state and S.E.
 template <int NT, int NS, typename PT>
 __device__ void force_range(JRef J, double *mol, int N, const Sc &c, const LdsR &S, int i0,
-                            int i1, double *xrow, double *site_dbg) {
+                            int i1, double *xrow, double *site_dbg, double *pt_sc = nullptr) {
+  PT_DECL
+  (void)pt_sc;
   constexpr int QB = (NS == 3) ? 0 : 1;
   constexpr int ST = Rec<NS>::STRIDE;
-  constexpr int NV = NVal<NS>::V;
   const PT *rec = (const PT *)S.region0;
   PairK<NS, PT> k;
   if (sizeof(PT) == 4) {
@@ -1473,6 +1497,7 @@ __device__ void force_range(JRef J, double *mol, int N, const Sc &c, const LdsR 
   }
   const int C = J.chunks;
   const int B = NT / C;  // molecules per round
+  const int NV = npart(NS, site_dbg != nullptr);
   const double dr = (J.cmode == 1) ? J.wall_length / (double)J.nfwall : 0.5 * J.wall_length / (double)J.nfwall;
   const double dri = 1.0 / dr;
   const int t = threadIdx.x;
@@ -1489,6 +1514,31 @@ __device__ void force_range(JRef J, double *mol, int N, const Sc &c, const LdsR 
       acc.corr[0] = acc.corr[1] = acc.corr[2] = 0.0;
       acc.vir[0] = acc.vir[1] = acc.vir[2] = 0.0;
       acc.lj = acc.clmb = 0.0;
+      const double q[4] = {mol[F_Q0 * N + i], mol[F_Q1 * N + i], mol[F_Q2 * N + i], mol[F_Q3 * N + i]};
+      // wall on the site-0 position (chunk 0 of each molecule), FP64: the
+      // table rows are fetched before the pair loop so their latency hides
+      // behind it (src/force_calculator.cxx:1494-1531)
+      const bool wall = cc == 0 && (J.cmode == 1 || J.cmode == 2);
+      double wx = 0.0, wy = 0.0, wz = 0.0, wu = 0.0, f0 = 0.0, f1 = 0.0, e0 = 0.0, e1 = 0.0;
+      int widx = 0;
+      if (wall) {
+        double b0[3];
+        body_to_space(q, (const double *)J.site_r[0], b0);
+        if (J.cmode == 1) {
+          wx = (mol[F_RX * N + i] - 0.5) * c.L[0] + b0[0];
+          wy = (mol[F_RY * N + i] - 0.5) * c.L[1] + b0[1];
+          wu = sqrt(wx * wx + wy * wy);
+        } else {
+          wz = (mol[F_RZ * N + i] - 0.5) * c.L[2] + b0[2];
+          wu = fabs(wz);
+        }
+        widx = (int)(wu / dr);
+        if (widx > J.nfwall - 2) widx = J.nfwall - 2;
+        f0 = J.fwall[2 * widx];
+        e0 = J.fwall[2 * widx + 1];
+        f1 = J.fwall[2 * (widx + 1)];
+        e1 = J.fwall[2 * (widx + 1) + 1];
+      }
       const PT *ri = rec + i * ST;
       PT gi[3] = {ri[0], ri[1], ri[2]};
       PT ai[NS][3];
@@ -1509,115 +1559,104 @@ __device__ void force_range(JRef J, double *mol, int N, const Sc &c, const LdsR 
         for (int u = 0; u < 3 + 3 * NS; u++) rj[u] = src[u];
         pair_acc<NS, QB, PT>(gi, ai, rj, k, acc);
       }
-      double *P = S.part;
-#pragma unroll
-      for (int a = 0; a < NS; a++)
-#pragma unroll
-        for (int d = 0; d < 3; d++) P[(3 * a + d) * NT + t] = acc.site[a][d];
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        P[(3 * NS + d) * NT + t] = acc.corr[d];
-        P[(3 * NS + 3 + d) * NT + t] = acc.vir[d];
-      }
-      P[(3 * NS + 6) * NT + t] = acc.lj;
-      P[(3 * NS + 7) * NT + t] = acc.clmb;
-    }
-    __syncthreads();
-    if (t < ie - ib) {
-      const int i = ib + t;
-      const double *P = S.part + t * C;
-      double v[NV];
-#pragma unroll
-      for (int u = 0; u < NV; u++) {
-        double s = P[u * NT];
-        for (int cc = 1; cc < C; cc++) s += P[u * NT + cc];
-        v[u] = s;
-      }
-      double site[NS][3];
-#pragma unroll
-      for (int a = 0; a < NS; a++)
-#pragma unroll
-        for (int d = 0; d < 3; d++) site[a][d] = v[3 * a + d];
-      const double q[4] = {mol[F_Q0 * N + i], mol[F_Q1 * N + i], mol[F_Q2 * N + i], mol[F_Q3 * N + i]};
-      double F[3];
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        double s = v[3 * NS + d];
-#pragma unroll
-        for (int a = 0; a < NS; a++) s += site[a][d];
-        F[d] = s;
-      }
       double e_wall = 0.0;
-      if (J.cmode == 1 || J.cmode == 2) {
-        double b0[3];
-        body_to_space(q, (const double *)J.site_r[0], b0);
+      if (wall) {
+        const double f = (f1 - f0) * dri * (wu - dr * widx) + f0;
+        e_wall = (e1 - e0) * dri * (wu - dr * widx) + e0;
         if (J.cmode == 1) {
-          const double x = (mol[F_RX * N + i] - 0.5) * c.L[0] + b0[0];
-          const double y = (mol[F_RY * N + i] - 0.5) * c.L[1] + b0[1];
-          const double r01 = sqrt(x * x + y * y);
-          int idx = (int)(r01 / dr);
-          if (idx > J.nfwall - 2) idx = J.nfwall - 2;
-          const double f = (J.fwall[2 * (idx + 1)] - J.fwall[2 * idx]) * dri * (r01 - dr * idx) + J.fwall[2 * idx];
-          const double e = (J.fwall[2 * (idx + 1) + 1] - J.fwall[2 * idx + 1]) * dri * (r01 - dr * idx) + J.fwall[2 * idx + 1];
-          site[0][0] += f * x;
-          site[0][1] += f * y;
-          F[0] += f * x;
-          F[1] += f * y;
-          e_wall = e;
+          acc.site[0][0] += f * wx;
+          acc.site[0][1] += f * wy;
         } else {
-          const double z = (mol[F_RZ * N + i] - 0.5) * c.L[2] + b0[2];
-          const double az = fabs(z);
-          int idx = (int)(az / dr);
-          if (idx > J.nfwall - 2) idx = J.nfwall - 2;
-          const double f = (J.fwall[2 * (idx + 1)] - J.fwall[2 * idx]) * dri * (az - dr * idx) + J.fwall[2 * idx];
-          const double e = (J.fwall[2 * (idx + 1) + 1] - J.fwall[2 * idx + 1]) * dri * (az - dr * idx) + J.fwall[2 * idx + 1];
-          site[0][2] += f * z;
-          F[2] += f * z;
-          e_wall = e;
+          acc.site[0][2] += f * wz;
         }
       }
-      double n4[4] = {0.0, 0.0, 0.0, 0.0};
+      // this item's COM force and A2M torque (src/molecules.cxx:401-439;
+      // linear in the site forces, so chunks can be summed afterwards)
+      double F[3], n4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        double sum = acc.corr[d];
+#pragma unroll
+        for (int a = 0; a < NS; a++) sum += acc.site[a][d];
+        F[d] = sum;
+      }
 #pragma unroll
       for (int a = 0; a < NS; a++) {
         double fb[3];
-        space_to_body(q, site[a], fb);
+        space_to_body(q, acc.site[a], fb);
         const double *br = (const double *)J.site_r[a];
         n4[0] += br[0] * fb[0] + br[1] * fb[1] + br[2] * fb[2];
         n4[1] += br[1] * fb[2] - br[2] * fb[1];
         n4[2] += br[2] * fb[0] - br[0] * fb[2];
         n4[3] += br[0] * fb[1] - br[1] * fb[0];
       }
-      const double ev[6] = {v[3 * NS + 6], v[3 * NS + 7], e_wall, v[3 * NS + 3], v[3 * NS + 4], v[3 * NS + 5]};
-      if (xrow) {
-        double *o = xrow + (size_t)i * XW;
-        o[0] = F[0];
-        o[1] = F[1];
-        o[2] = F[2];
-        o[3] = n4[0];
-        o[4] = n4[1];
-        o[5] = n4[2];
-        o[6] = n4[3];
-#pragma unroll
-        for (int u = 0; u < 6; u++) o[7 + u] = ev[u];
-      } else {
-        mol[F_FX * N + i] = F[0];
-        mol[F_FY * N + i] = F[1];
-        mol[F_FZ * N + i] = F[2];
-        mol[F_N0 * N + i] = n4[0];
-        mol[F_N1 * N + i] = n4[1];
-        mol[F_N2 * N + i] = n4[2];
-        mol[F_N3 * N + i] = n4[3];
-#pragma unroll
-        for (int u = 0; u < 6; u++) S.E[u * N + i] = ev[u];
-      }
+      double *P = S.part;
+      P[0 * NT + t] = F[0];
+      P[1 * NT + t] = F[1];
+      P[2 * NT + t] = F[2];
+      P[3 * NT + t] = n4[0];
+      P[4 * NT + t] = n4[1];
+      P[5 * NT + t] = n4[2];
+      P[6 * NT + t] = n4[3];
+      P[7 * NT + t] = acc.vir[0];
+      P[8 * NT + t] = acc.vir[1];
+      P[9 * NT + t] = acc.vir[2];
+      P[10 * NT + t] = acc.lj;
+      P[11 * NT + t] = acc.clmb;
+      P[12 * NT + t] = e_wall;
       if (site_dbg) {
 #pragma unroll
         for (int a = 0; a < NS; a++)
 #pragma unroll
-          for (int d = 0; d < 3; d++) site_dbg[(i * NS + a) * 3 + d] = site[a][d];
+          for (int d = 0; d < 3; d++) P[(NPART + 3 * a + d) * NT + t] = acc.site[a][d];
       }
     }
     __syncthreads();
+    PT_MARK(12, pt_sc);
+    if (t < ie - ib) {
+      const int i = ib + t;
+      const double *P = S.part + t * C;
+      // chunks in order for every value; chunk-outer so each chunk's loads
+      // issue together
+      double v[NPART];
+#pragma unroll
+      for (int u = 0; u < NPART; u++) v[u] = P[u * NT];
+      for (int cc = 1; cc < C; cc++) {
+        double w[NPART];
+#pragma unroll
+        for (int u = 0; u < NPART; u++) w[u] = P[u * NT + cc];
+#pragma unroll
+        for (int u = 0; u < NPART; u++) v[u] += w[u];
+      }
+      // F, n, then lj, clmb, wall, vir
+      const double ev[6] = {v[10], v[11], v[12], v[7], v[8], v[9]};
+      if (xrow) {
+        double *o = xrow + (size_t)i * XW;
+#pragma unroll
+        for (int u = 0; u < 7; u++) o[u] = v[u];
+#pragma unroll
+        for (int u = 0; u < 6; u++) o[7 + u] = ev[u];
+      } else {
+        mol[F_FX * N + i] = v[0];
+        mol[F_FY * N + i] = v[1];
+        mol[F_FZ * N + i] = v[2];
+        mol[F_N0 * N + i] = v[3];
+        mol[F_N1 * N + i] = v[4];
+        mol[F_N2 * N + i] = v[5];
+        mol[F_N3 * N + i] = v[6];
+#pragma unroll
+        for (int u = 0; u < 6; u++) S.E[u * N + i] = ev[u];
+      }
+      if (site_dbg) {
+        for (int u = NPART; u < NV; u++) {
+          double sum = P[u * NT];
+          for (int cc = 1; cc < C; cc++) sum += P[u * NT + cc];
+          site_dbg[i * NS * 3 + (u - NPART)] = sum;
+        }
+      }
+    }
+    __syncthreads();
+    PT_MARK(13, pt_sc);
   }
 }
 
@@ -1634,7 +1673,7 @@ __device__ bool forces_resident(JRef J, double *mol, int N, Sc &c, const LdsR &S
   PT_MARK(4, pt_sc);
   if (g.parts > 1) {
     double *xrow = g.xbuf + (size_t)(g.epoch & 1u) * N * XW;
-    force_range<NT, NS, PT>(J, mol, N, c, S, i0, i1, xrow, site_dbg);
+    force_range<NT, NS, PT>(J, mol, N, c, S, i0, i1, xrow, site_dbg, pt_sc);
     PT_MARK(5, pt_sc);
     if (!group_wait<NT>(g, S.flag, pt_sc)) return false;
     PT_MARK(6, pt_sc);
@@ -1686,7 +1725,7 @@ template <int NT, int NS, typename PT>
 __device__ void run_resident(const LaunchArgs &a, const DescD &R, JRef J, char *smem,
                              Group &g) {
   const int N = J.nmol;
-  LdsR S = carve_resident<NT, NS>(smem, N, J.precision);
+  LdsR S = carve_resident<NT, NS>(smem, N, J.precision, R.site_f != nullptr);
   double *mol = S.mol;
   int *mid = S.mid;
   for (int k = threadIdx.x; k < NFIELD * N; k += NT) mol[k] = gen(R.mol)[k];
@@ -1716,13 +1755,13 @@ __device__ void run_resident(const LaunchArgs &a, const DescD &R, JRef J, char *
 
   if (a.mode == MODE_INIT) {
     // InitializeProperty (src/molecules.h:260-270)
-    rank_sort_resident<NT>(J, mol, mid, N, Laxis0, 0.0, S, win_dbg);
+    rank_sort_resident<NT, NS>(J, mol, mid, N, Laxis0, 0.0, S, win_dbg);
     if (!forces_resident<NT, NS, PT>(J, mol, N, c, S, g, i0, i1, site_dbg)) return;
     c.gkT = 6.0 * (double)N * c.T;
     c.H0 = total_energy<NT>(J, mol, N, c, S.red);
     calc_properties<NT>(J, mol, N, c, sc_out, S.red);
   } else if (a.mode == MODE_FORCE) {
-    if (a.do_sort) rank_sort_resident<NT>(J, mol, mid, N, Laxis0, a.margin, S, win_dbg);
+    if (a.do_sort) rank_sort_resident<NT, NS>(J, mol, mid, N, Laxis0, a.margin, S, win_dbg);
     else window_from_debug<NT>(gen(R.win), N, S);
     if (!forces_resident<NT, NS, PT>(J, mol, N, c, S, g, i0, i1, site_dbg)) return;
     calc_properties<NT>(J, mol, N, c, sc_out, S.red);
@@ -1755,7 +1794,7 @@ __device__ void run_resident(const LaunchArgs &a, const DescD &R, JRef J, char *
       first_half<NT>(J, mol, N, c, S.red);
       PT_MARK(0, pt);
       if ((st % J.sort_interval) == 0)
-        rank_sort_resident<NT>(J, mol, mid, N, J.sort_axis == 0 ? c.L[0] : c.L[2], J.margin, S, win_dbg);
+        rank_sort_resident<NT, NS>(J, mol, mid, N, J.sort_axis == 0 ? c.L[0] : c.L[2], J.margin, S, win_dbg);
       PT_MARK(1, pt);
       if (!forces_resident<NT, NS, PT>(J, mol, N, c, S, g, i0, i1, site_dbg, pt)) return;
       PT_MARK(2, pt);
@@ -2010,7 +2049,9 @@ static const void *resident_fn(int nsite, int prec) {
 
 int resident_grid(int nrep, int parts) { return parts <= 1 ? nrep : 8 * parts * ((nrep + 7) / 8); }
 
-int resident_lds(int nmol, int nsite, int prec) { return resident_lds_bytes(nmol, nsite, prec, 256); }
+int resident_lds(int nmol, int nsite, int prec, int site_dbg) {
+  return resident_lds_bytes(nmol, nsite, prec, 256, site_dbg != 0);
+}
 
 hipError_t launch_resident(const LaunchArgs &a, int nsite, int prec, int lds, hipStream_t st) {
   if (a.nrep <= 0) return hipSuccess;
