@@ -28,6 +28,24 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3  # MI355X dense FP32 vector peak
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_resident_kernel_tube144_tip4p_64rep.json")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE
+    / WRITE_SIZE runs, each scaled by the same counter on a copy of known
+    size; None when no measurement of this kernel is on file."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        m = json.load(f)
+    if m.get("kernel") != kernel:
+        return None
+    kib = 1024.0
+    true_kib = m["copy_bytes"] / kib
+    fetch = np.mean(m["fetch_size_kib_per_launch"]) / (m["copy_fetch_size_kib"] / true_kib)
+    write = np.mean(m["write_size_kib_per_launch"]) / (m["copy_write_size_kib"] / true_kib)
+    return float((fetch + write) * kib)
 
 
 def cpu_baseline(system, T, P, budget_s=30.0):
@@ -90,6 +108,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--workload", default="tube64", choices=["tube64", "uneven", "served", "precision"],
+                    help="tube64: the headline line; the others print one line of their own study")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run 3 device-to-device copies of this many MiB (PMC counter calibration)")
     args = ap.parse_args()
@@ -110,6 +130,12 @@ def main():
 
     import remd_amd as ra
     from remd_amd import inputs, workloads
+
+    if args.workload != "tube64":
+        study = {"uneven": workloads.uneven_batch_study, "served": workloads.served_study,
+                 "precision": workloads.precision_study}[args.workload]
+        print(json.dumps({"workload": args.workload, "n_gpus": 1, "data": "synthetic", **study()}))
+        return
 
     system = inputs.config_tube(seed=2)
     system = workloads.relax(system)
@@ -213,7 +239,7 @@ def main():
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP32_TFLOPS, 5),
-            "traffic": None,
+            "traffic": measured_traffic(kname),
             "flops_per_launch": flops_per_launch,
             "kernel_ms": round(kernel_ms, 4),
         },

@@ -203,5 +203,7 @@ def poisson_stream(seed=5, njobs=40, rate=20.0, intervals=50, nmol_hi=2048, rela
         s = inputs.served_job(rng, nmol_hi=nmol_hi)
         if relax:
             s = workloads.relax(s)
+            print(f"job {k}: {s.model} confined={s.confined} nmol={s.nmol} replicas={s.nreplica}",
+                  file=__import__("sys").stderr, flush=True)
         reqs.append(Request(job_id=k, arrival=t, system=s, intervals=intervals))
     return reqs
