@@ -172,8 +172,20 @@ int remd_batch_add(remd_batch *b, remd_gpu *g);
 int remd_batch_remove(remd_batch *b, remd_gpu *g);
 /* Stage conditions for a job's next batched interval. */
 int remd_gpu_set_conditions(remd_gpu *g, const double *T, const double *P);
-/* One interval for every job of the batch, one kernel launch. */
+/* One interval for every job of the batch: every resident-path replica in
+ * one kernel launch (split over 1..8 workgroups each by the batch's packing
+ * plan), then one launch per global-state kernel class.  The first
+ * intervals after the batch changes time the candidate plans; results do
+ * not depend on the plan. */
 int remd_batch_execute(remd_batch *b);
+/* Time candidate packing plans on the first intervals (default on); off:
+ * keep the cost model's best plan. */
+int remd_batch_set_autotune(remd_batch *b, int on);
+/* Fix the packing plan: workgroups per replica for each job, in the order
+ * the jobs were added (1..min(8, nmol / 8) for a resident-path job, 1 for
+ * a global-state job); parts == NULL restores the automatic plan.  Adding
+ * or removing a job also restores it. */
+int remd_batch_set_parts(remd_batch *b, const int *parts, int n);
 int remd_batch_last_kernel_ms(remd_batch *b, double *ms);
 void remd_batch_destroy(remd_batch *b);
 /* Device bytes a job of this shape keeps resident (admission control). */

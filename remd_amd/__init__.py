@@ -97,6 +97,8 @@ def lib():
     L.remd_batch_add.argtypes = [C.c_void_p, C.c_void_p]
     L.remd_batch_remove.argtypes = [C.c_void_p, C.c_void_p]
     L.remd_batch_execute.argtypes = [C.c_void_p]
+    L.remd_batch_set_autotune.argtypes = [C.c_void_p, C.c_int]
+    L.remd_batch_set_parts.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.c_int]
     L.remd_batch_last_kernel_ms.argtypes = [C.c_void_p, _dp]
     L.remd_batch_destroy.argtypes = [C.c_void_p]
     L.remd_batch_destroy.restype = None
@@ -324,6 +326,24 @@ class Batch:
 
     def execute(self):
         _check(lib().remd_batch_execute(self.h))
+
+    def set_autotune(self, on: bool):
+        """Time candidate packing plans on the first intervals after the
+        batch changes (default), or keep the cost model's best plan."""
+        _check(lib().remd_batch_set_autotune(self.h, int(bool(on))))
+
+    def set_parts(self, parts=None):
+        """Fix the workgroups per replica of each job (in add order); None
+        restores the automatic plan."""
+        if parts is None:
+            _check(lib().remd_batch_set_parts(self.h, None, 0))
+        else:
+            arr = (C.c_int * len(parts))(*[int(p) for p in parts])
+            _check(lib().remd_batch_set_parts(self.h, arr, len(parts)))
+
+    def parts(self):
+        """Workgroups per replica each job got in the last interval."""
+        return [j.launch_path()[1] for j in self.jobs]
 
     def last_kernel_ms(self):
         ms = C.c_double()

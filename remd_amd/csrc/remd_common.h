@@ -98,7 +98,10 @@ struct LaunchArgs {
   int mode;
   int do_sort;                  // MODE_FORCE only
   double margin;                // MODE_FORCE only
-  int parts;                    // resident path: workgroups per replica
+  int parts;                    // resident path: workgroups per replica (blk == nullptr)
+  const unsigned *blk = nullptr; // resident batch launch: per-workgroup entry (rep << 12 |
+                                 // variant << 8 | parts << 4 | part), ~0u for an idle workgroup
+  int nblk = 0;                  // workgroups of a resident batch launch
   unsigned long long *sync;     // resident path, parts > 1: nrep group words, zeroed (low 32
                                 // bits: arrivals; high 32: XCD histogram of the parts)
   unsigned *abort;              // resident path, parts > 1: abort word, zeroed

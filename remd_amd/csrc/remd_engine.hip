@@ -22,6 +22,8 @@ hipError_t launch_interval(const LaunchArgs &a, int nt, int nsite, int prec, int
 hipError_t set_kernel_lds_limit(int bytes);
 hipError_t launch_resident(const LaunchArgs &a, int nsite, int prec, int lds, hipStream_t st);
 hipError_t resident_occupancy(int nsite, int prec, int lds, int *per_cu);
+hipError_t launch_resident_batch(const LaunchArgs &a, int lds, hipStream_t st);
+hipError_t resident_batch_occupancy(int lds, int *per_cu);
 int resident_lds(int nmol, int nsite, int prec, int site_dbg);
 hipError_t launch_wall_tube(int nfwall, double R, double sgm06, double coefE, double *fw, hipStream_t st);
 hipError_t launch_wall_slit(int nfwall, double H, double sgm, double eps, double rho, double *fw,
@@ -107,6 +109,7 @@ struct remd_gpu {
   int nt = 256, lds = 0;
   bool resident = false;     // state in LDS for the launch (resident_kernel)
   int capacity = 0;          // resident workgroups the device holds at once
+  double wfrac = 1.0;        // window fraction of the sort axis, ~2 (rcut + margin) / L
   int last_parts = 1;
   unsigned long long *d_sync = nullptr;  // nrep group words + the abort word
   double *d_xbuf = nullptr;    // nrep x 2 x N x XW exchange rows
@@ -135,28 +138,47 @@ struct remd_gpu {
   ExchangeJob *d_xjob = nullptr;
 };
 
-// Kernel class = (path, sites, pair precision); one launch per class, each
-// on its own stream.  Resident classes run concurrently and share the
-// co-resident workgroup slots; global-state classes start after them, so a
-// long global-state launch never holds the CUs a waiting part needs.
-constexpr int NCLASS = 12;
-static int kclass(bool resident, int nsite, int prec) { return (resident ? 6 : 0) + (nsite - 3) * 2 + prec; }
+// Every resident replica of a batch runs in ONE launch of
+// resident_batch_kernel (any water model and precision).  Global-state
+// replicas are launched per class = (sites, pair precision), each class on
+// its own stream, after the resident launch, so a long global-state launch
+// never holds the CUs a waiting part needs.
+constexpr int NCLASS = 6;
+static int kclass(int nsite, int prec) { return (nsite - 3) * 2 + prec; }
 
 struct remd_batch {
   std::vector<remd_gpu *> jobs;
-  ReplicaDesc *d_table = nullptr;
+  ReplicaDesc *d_table = nullptr;  // resident rows (heaviest first), then the global-state classes
   int table_cap = 0;
   int ntab = 0;
   bool dirty = true;
   int nt = 256;
+  // resident rows
+  int res_n = 0, res_lds = 0, res_capacity = 0;
+  std::vector<int> row_job;  // resident row -> index into jobs
+  // global-state classes
   int cls_off[NCLASS] = {0}, cls_n[NCLASS] = {0}, cls_lds[NCLASS] = {0};
-  int res_capacity = 0, res_minN = 0, res_reps = 0, parts = 1;
-  unsigned long long *d_sync = nullptr;  // ntab group words + the abort word
+  // packing plan: workgroups per replica of each job, and its block table
+  std::vector<int> plan;
+  unsigned *d_blk = nullptr;
+  int blk_cap = 0, nblk = 0;
+  bool split = false;  // some replica of the plan has parts > 1
+  // autotuner: candidate plans (best estimate first), each timed on one
+  // interval after an untimed one; the fastest is kept until the batch
+  // changes.  Results do not depend on the plan.
+  std::vector<std::vector<int>> cands;
+  std::vector<double> cand_ms;
+  int cand_cur = 0;
+  int warm = 0;
+  bool tuned = true;
+  int autotune = 1;
+  std::vector<int> forced;  // plan fixed by remd_batch_set_parts (empty: automatic)
+  unsigned long long *d_sync = nullptr;  // res_n group words + the abort word
   unsigned *h_abort = nullptr;
   hipStream_t stream = nullptr;
   hipStream_t cstream[NCLASS] = {nullptr};
   hipEvent_t cdone[NCLASS] = {nullptr};
-  hipEvent_t ev0 = nullptr, ev1 = nullptr, evres = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, efork = nullptr;
   double last_ms = 0.0;
 };
 
@@ -460,6 +482,7 @@ extern "C" int remd_gpu_create(remd_gpu **out, const remd_gpu_desc *d, const rem
       return fail(REMD_EHIP, "occupancy query for the resident kernel failed");
     }
     g->capacity = per_cu * ncu;
+    g->wfrac = wf;
     // chunks: the (molecule, chunk) items of one part fill its 256 lanes at
     // the part count this job gets alone; fixed for the job's lifetime, so
     // results do not depend on the part count of a later launch
@@ -1041,15 +1064,17 @@ extern "C" int remd_batch_create(remd_batch **out) {
   hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&b->ev0);
   if (e == hipSuccess) e = hipEventCreate(&b->ev1);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->evres, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->efork, hipEventDisableTiming);
   for (int c = 0; c < NCLASS && e == hipSuccess; c++) {
     e = hipStreamCreateWithFlags(&b->cstream[c], hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->cdone[c], hipEventDisableTiming);
   }
+  if (e == hipSuccess) e = hipHostMalloc(&b->h_abort, sizeof(unsigned), hipHostMallocDefault);
   if (e != hipSuccess) {
-    delete b;
+    remd_batch_destroy(b);
     return fail(REMD_EHIP, hipGetErrorString(e));
   }
+  *b->h_abort = 0u;
   *out = b;
   return 0;
 }
@@ -1071,36 +1096,208 @@ extern "C" int remd_batch_remove(remd_batch *b, remd_gpu *g) {
   return 0;
 }
 
-// Table of every replica of the batch grouped by kernel class, heaviest
-// first within a class (workgroups are dispatched in blockIdx order, so long
-// replicas start early).
-static int rebuild_table(remd_batch *b) {
-  std::vector<std::pair<std::pair<int, double>, ReplicaDesc>> rows;
-  for (int c = 0; c < NCLASS; c++) b->cls_lds[c] = b->cls_n[c] = b->cls_off[c] = 0;
-  b->res_capacity = 0;
-  b->res_minN = 0;
-  b->res_reps = 0;
-  for (remd_gpu *g : b->jobs) {
-    refresh_reps(g);
-    const int c = kclass(g->resident, g->NS, g->prm.precision);
-    const double w = (double)g->N * g->N * g->NS * g->NS * std::max(1, g->desc.interval);
-    for (int r = 0; r < g->nrep; r++) rows.push_back({{c, -w}, g->h_reps[r]});
-    b->cls_lds[c] = std::max(b->cls_lds[c], g->lds);
-    b->cls_n[c] += g->nrep;
-    if (g->resident) {
-      b->res_capacity = b->res_capacity ? std::min(b->res_capacity, g->capacity) : g->capacity;
-      b->res_minN = b->res_minN ? std::min(b->res_minN, g->N) : g->N;
-      b->res_reps += g->nrep;
+extern "C" int remd_batch_set_autotune(remd_batch *b, int on) {
+  if (!b) return fail(REMD_EINVAL, "null batch");
+  b->autotune = on != 0;
+  b->dirty = true;
+  return 0;
+}
+
+extern "C" int remd_batch_set_parts(remd_batch *b, const int *parts, int n) {
+  if (!b) return fail(REMD_EINVAL, "null batch");
+  if (!parts) {
+    b->forced.clear();
+  } else {
+    if (n != (int)b->jobs.size()) return fail(REMD_EINVAL, "one part count per job of the batch");
+    for (int k = 0; k < n; k++) {
+      const remd_gpu *g = b->jobs[k];
+      const int pmax = g->resident ? std::min(GMAX, std::max(1, g->N / 8)) : 1;
+      if (parts[k] < 1 || parts[k] > pmax)
+        return fail(REMD_EINVAL, "job " + std::to_string(k) + ": parts must be 1.." + std::to_string(pmax));
+    }
+    b->forced.assign(parts, parts + n);
+  }
+  b->dirty = true;
+  return 0;
+}
+
+// Relative time of one MD step of one replica of g split over p
+// workgroups: the integration every part repeats (O(N)), the rank sort
+// (O(N^2) compares over 256 lanes), the pair work over the sort-axis
+// window shared out among the parts (twice as dear in FP64), and for p > 1
+// the exchange rows and the group barrier.  Coefficients are per-phase
+// timings of the resident kernel on 144 TIP4P in a nanotube, in
+// microseconds; the autotuner corrects what this model gets wrong.
+static double step_cost(const remd_gpu *g, int p) {
+  const double N = g->N, ns = g->NS;
+  const double pair = 2.2e-4 * N * N * g->wfrac * ns * ns * (g->prm.precision ? 2.0 : 1.0);
+  return 1.3e-2 * N * ns + 6.6e-5 * N * N + pair / p + (p > 1 ? 2.6e-2 * N : 0.0);
+}
+
+static double interval_cost(const remd_gpu *g, int p) { return step_cost(g, p) * std::max(1, g->desc.interval); }
+
+static unsigned variant_of(const remd_gpu *g) { return (unsigned)((g->NS - 3) * 2 + g->prm.precision); }
+
+// Estimated time of one batched interval under plan P: the split replicas
+// hold their workgroups for their (longest) interval; the single-part
+// replicas fill the remaining slots meanwhile and every slot afterwards.
+static double plan_time(const remd_batch *b, const std::vector<int> &P) {
+  double tg = 0.0, w1 = 0.0, m1 = 0.0;
+  double used = 0.0;
+  for (size_t k = 0; k < b->jobs.size(); k++) {
+    const remd_gpu *g = b->jobs[k];
+    if (!g->resident) continue;
+    const double t = interval_cost(g, P[k]);
+    if (P[k] > 1) {
+      tg = std::max(tg, t);
+      used += (double)g->nrep * P[k];
+    } else {
+      w1 += g->nrep * t;
+      m1 = std::max(m1, t);
     }
   }
-  b->parts = pick_parts(b->res_capacity, b->res_reps, b->res_minN);
-  std::stable_sort(rows.begin(), rows.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
-  int off = 0;
+  const double cap = std::max(1, b->res_capacity), fr = cap - used;
+  const double T = fr * tg >= w1 ? tg : tg + (w1 - fr * tg) / cap;
+  return std::max(T, m1);
+}
+
+// Block table of plan P: first the parts of every split replica, packed
+// into the 8 block columns (block index mod 8, one XCD each) with all parts
+// of a replica in one column, more parts first; then the single-part
+// replicas, heaviest first.  False when the split parts need more than
+// capacity / 8 rows: they must all be resident at once.
+static bool layout_plan(const remd_batch *b, const std::vector<int> &P, std::vector<unsigned> *out) {
+  std::vector<unsigned> col[8];
+  int h[8] = {0};
+  std::vector<int> split, single;
+  for (int r = 0; r < b->res_n; r++) (P[b->row_job[r]] > 1 ? split : single).push_back(r);
+  std::stable_sort(split.begin(), split.end(),
+                   [&](int x, int y) { return P[b->row_job[x]] > P[b->row_job[y]]; });
+  for (int r : split) {
+    const int p = P[b->row_job[r]];
+    int x = 0;
+    for (int k = 1; k < 8; k++)
+      if (h[k] < h[x]) x = k;
+    const unsigned v = variant_of(b->jobs[b->row_job[r]]);
+    for (int q = 0; q < p; q++) col[x].push_back((unsigned)r << 12 | v << 8 | (unsigned)p << 4 | (unsigned)q);
+    h[x] += p;
+  }
+  int H = 0;
+  for (int x = 0; x < 8; x++) H = std::max(H, h[x]);
+  if (H > b->res_capacity / 8) return false;
+  if (out) {
+    out->clear();
+    for (int s = 0; s < H; s++)
+      for (int x = 0; x < 8; x++) out->push_back(s < h[x] ? col[x][s] : ~0u);
+    for (int r : single) out->push_back((unsigned)r << 12 | variant_of(b->jobs[b->row_job[r]]) << 8 | 1u << 4);
+  }
+  return true;
+}
+
+// Candidate plans: from one workgroup per replica, repeatedly give the job
+// whose replicas take longest one more part, as long as the split parts
+// still fit; keep the (up to 4) plans estimated within 25% of the best.
+static void make_candidates(remd_batch *b) {
+  const size_t nj = b->jobs.size();
+  std::vector<int> P(nj, 1), capped(nj, 0);
+  std::vector<std::pair<double, std::vector<int>>> seen{{plan_time(b, P), P}};
+  for (;;) {
+    int k = -1;
+    double worst = -1.0;
+    for (size_t j = 0; j < nj; j++) {
+      const remd_gpu *g = b->jobs[j];
+      if (!g->resident || capped[j] || P[j] >= std::min(GMAX, std::max(1, g->N / 8))) continue;
+      const double t = interval_cost(g, P[j]);
+      if (t > worst) {
+        worst = t;
+        k = (int)j;
+      }
+    }
+    if (k < 0) break;
+    std::vector<int> Q = P;
+    Q[k]++;
+    if (!layout_plan(b, Q, nullptr)) {
+      capped[k] = 1;
+      continue;
+    }
+    P = Q;
+    seen.push_back({plan_time(b, P), P});
+  }
+  std::stable_sort(seen.begin(), seen.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+  b->cands.clear();
+  for (const auto &c : seen) {
+    if (b->cands.size() >= 4 || c.first > 1.25 * seen[0].first) break;
+    b->cands.push_back(c.second);
+  }
+}
+
+static int apply_plan(remd_batch *b, const std::vector<int> &P) {
+  std::vector<unsigned> blk;
+  if (!layout_plan(b, P, &blk)) return fail(REMD_EINVAL, "batch plan does not fit the device");
+  const int n = (int)blk.size();
+  if (n > b->blk_cap) {
+    hipFree(b->d_blk);
+    b->d_blk = nullptr;
+    HIPCHK(hipMalloc(&b->d_blk, sizeof(unsigned) * std::max(n, 1)));
+    b->blk_cap = n;
+  }
+  if (n) HIPCHK(hipMemcpy(b->d_blk, blk.data(), sizeof(unsigned) * n, hipMemcpyHostToDevice));
+  b->nblk = n;
+  b->plan = P;
+  b->split = false;
+  for (size_t k = 0; k < b->jobs.size(); k++)
+    if (b->jobs[k]->resident && P[k] > 1) b->split = true;
+  return 0;
+}
+
+// Table of every replica of the batch: the resident rows heaviest first,
+// then the global-state rows by class, heaviest first within a class
+// (workgroups are dispatched in block order, so long replicas start early).
+static int rebuild_table(remd_batch *b) {
+  struct Row {
+    int cls;  // -1: resident
+    double w;
+    int job;
+    ReplicaDesc d;
+  };
+  std::vector<Row> rows;
+  for (int c = 0; c < NCLASS; c++) b->cls_lds[c] = b->cls_n[c] = b->cls_off[c] = 0;
+  b->res_n = b->res_lds = 0;
+  for (size_t k = 0; k < b->jobs.size(); k++) {
+    remd_gpu *g = b->jobs[k];
+    refresh_reps(g);
+    const int c = g->resident ? -1 : kclass(g->NS, g->prm.precision);
+    const double w = interval_cost(g, 1);
+    for (int r = 0; r < g->nrep; r++) rows.push_back({c, -w, (int)k, g->h_reps[r]});
+    if (g->resident) {
+      b->res_lds = std::max(b->res_lds, g->lds);
+      b->res_n += g->nrep;
+    } else {
+      b->cls_lds[c] = std::max(b->cls_lds[c], g->lds);
+      b->cls_n[c] += g->nrep;
+    }
+  }
+  if (b->res_n >= (1 << 20)) return fail(REMD_EINVAL, "too many resident replicas in one batch");
+  if (b->res_lds > device_lds_limit()) return fail(REMD_EINVAL, "batch LDS exceeds the device limit");
+  b->res_capacity = 0;
+  if (b->res_n) {
+    int per_cu = 0, ncu = 0, dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(resident_batch_occupancy(b->res_lds, &per_cu));
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    if (per_cu < 1) return fail(REMD_EINVAL, "the resident batch kernel does not fit a CU");
+    b->res_capacity = per_cu * ncu;
+  }
+  std::stable_sort(rows.begin(), rows.end(),
+                   [](const Row &x, const Row &y) { return x.cls != y.cls ? x.cls < y.cls : x.w < y.w; });
+  int off = b->res_n;
   for (int c = 0; c < NCLASS; c++) {
     b->cls_off[c] = off;
     off += b->cls_n[c];
   }
   const int n = (int)rows.size();
+  b->row_job.resize(b->res_n);
+  for (int r = 0; r < b->res_n; r++) b->row_job[r] = rows[r].job;
   if (n > b->table_cap) {
     hipFree(b->d_table);
     b->d_table = nullptr;
@@ -1109,14 +1306,27 @@ static int rebuild_table(remd_batch *b) {
   }
   hipFree(b->d_sync);
   b->d_sync = nullptr;
-  HIPCHK(hipMalloc(&b->d_sync, sizeof(unsigned long long) * (n + 1)));
-  if (!b->h_abort) HIPCHK(hipHostMalloc(&b->h_abort, sizeof(unsigned), hipHostMallocDefault));
+  HIPCHK(hipMalloc(&b->d_sync, sizeof(unsigned long long) * (b->res_n + 1)));
   *b->h_abort = 0u;
   std::vector<ReplicaDesc> flat(n);
-  for (int k = 0; k < n; k++) flat[k] = rows[k].second;
+  for (int k = 0; k < n; k++) flat[k] = rows[k].d;
   if (n) HIPCHK(hipMemcpyAsync(b->d_table, flat.data(), sizeof(ReplicaDesc) * n, hipMemcpyHostToDevice, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
   b->ntab = n;
+  if (!b->forced.empty() && b->forced.size() == b->jobs.size()) {
+    if (!layout_plan(b, b->forced, nullptr))
+      return fail(REMD_EINVAL, "the fixed parts need more workgroups than the device holds at once");
+    b->cands.assign(1, b->forced);
+  } else {
+    b->forced.clear();
+    make_candidates(b);
+  }
+  b->cand_ms.assign(b->cands.size(), 0.0);
+  b->cand_cur = 0;
+  b->warm = 1;
+  b->tuned = !b->autotune || b->cands.size() <= 1;
+  int rc = apply_plan(b, b->cands[0]);
+  if (rc) return rc;
   b->dirty = false;
   return 0;
 }
@@ -1132,49 +1342,83 @@ extern "C" int remd_batch_execute(remd_batch *b) {
   }
   // conditions were staged on each job's stream; order them before the launch
   for (remd_gpu *g : b->jobs) HIPCHK(hipStreamSynchronize(g->stream));
-  if (b->parts > 1) HIPCHK(hipMemsetAsync(b->d_sync, 0, sizeof(unsigned long long) * (b->ntab + 1), b->stream));
+  if (b->split) HIPCHK(hipMemsetAsync(b->d_sync, 0, sizeof(unsigned long long) * (b->res_n + 1), b->stream));
+  for (size_t k = 0; k < b->jobs.size(); k++) b->jobs[k]->last_parts = b->jobs[k]->resident ? b->plan[k] : 1;
   HIPCHK(hipEventRecord(b->ev0, b->stream));
-  // resident classes first (concurrently), then the global-state classes
-  for (int pass = 0; pass < 2; pass++) {
-    if (pass == 1) HIPCHK(hipEventRecord(b->evres, b->stream));
-    for (int c = 0; c < NCLASS; c++) {
-      const bool res = c >= 6;
-      if (!b->cls_n[c] || res != (pass == 0)) continue;
-      if (b->cls_lds[c] > device_lds_limit()) return fail(REMD_EINVAL, "batch LDS exceeds the device limit");
-      HIPCHK(hipStreamWaitEvent(b->cstream[c], pass == 0 ? b->ev0 : b->evres, 0));
-      LaunchArgs a;
-      a.reps = b->d_table + b->cls_off[c];
-      a.nrep = b->cls_n[c];
-      a.mode = MODE_INTERVAL;
-      a.do_sort = 0;
-      a.margin = 0.0;
-      a.parts = res ? b->parts : 1;
-      a.same_xcd_ok = same_xcd_barrier_enabled();
-      a.same_xcd_mode = same_xcd_barrier_mode();
-      a.sync = b->d_sync + b->cls_off[c];
-      a.abort = (unsigned *)(b->d_sync + b->ntab);
-      const int nsite = 3 + (c % 6) / 2, prec = c % 2;
-      if (res) HIPCHK(launch_resident(a, nsite, prec, b->cls_lds[c], b->cstream[c]));
-      else HIPCHK(launch_interval(a, b->nt, nsite, prec, b->cls_lds[c], b->cstream[c]));
-      HIPCHK(hipEventRecord(b->cdone[c], b->cstream[c]));
-      HIPCHK(hipStreamWaitEvent(b->stream, b->cdone[c], 0));
+  if (b->nblk) {
+    LaunchArgs a;
+    a.reps = b->d_table;
+    a.nrep = b->res_n;
+    a.mode = MODE_INTERVAL;
+    a.do_sort = 0;
+    a.margin = 0.0;
+    a.parts = 1;
+    a.blk = b->d_blk;
+    a.nblk = b->nblk;
+    a.same_xcd_ok = same_xcd_barrier_enabled();
+    a.same_xcd_mode = same_xcd_barrier_mode();
+    a.sync = b->d_sync;
+    a.abort = (unsigned *)(b->d_sync + b->res_n);
+    HIPCHK(launch_resident_batch(a, b->res_lds, b->stream));
+  }
+  // then the global-state classes, concurrently
+  bool fork = false;
+  for (int c = 0; c < NCLASS; c++) {
+    if (!b->cls_n[c]) continue;
+    if (b->cls_lds[c] > device_lds_limit()) return fail(REMD_EINVAL, "batch LDS exceeds the device limit");
+    if (!fork) {
+      HIPCHK(hipEventRecord(b->efork, b->stream));
+      fork = true;
     }
+    HIPCHK(hipStreamWaitEvent(b->cstream[c], b->efork, 0));
+    LaunchArgs a;
+    a.reps = b->d_table + b->cls_off[c];
+    a.nrep = b->cls_n[c];
+    a.mode = MODE_INTERVAL;
+    a.do_sort = 0;
+    a.margin = 0.0;
+    a.parts = 1;
+    a.same_xcd_ok = 0;
+    a.same_xcd_mode = 0;
+    a.sync = nullptr;
+    a.abort = nullptr;
+    const int nsite = 3 + c / 2, prec = c % 2;
+    HIPCHK(launch_interval(a, b->nt, nsite, prec, b->cls_lds[c], b->cstream[c]));
+  }
+  // join
+  for (int c = 0; c < NCLASS; c++) {
+    if (!b->cls_n[c]) continue;
+    HIPCHK(hipEventRecord(b->cdone[c], b->cstream[c]));
+    HIPCHK(hipStreamWaitEvent(b->stream, b->cdone[c], 0));
   }
   HIPCHK(hipEventRecord(b->ev1, b->stream));
-  if (b->parts > 1)
-    HIPCHK(hipMemcpyAsync(b->h_abort, b->d_sync + b->ntab, sizeof(unsigned), hipMemcpyDeviceToHost, b->stream));
-  for (remd_gpu *g : b->jobs) {
-    g->last_parts = g->resident ? b->parts : 1;
+  if (b->split)
+    HIPCHK(hipMemcpyAsync(b->h_abort, b->d_sync + b->res_n, sizeof(unsigned), hipMemcpyDeviceToHost, b->stream));
+  for (remd_gpu *g : b->jobs)
     HIPCHK(hipMemcpyAsync(g->h_sc, g->d_sc, sizeof(double) * NSC * g->nrep, hipMemcpyDeviceToHost, b->stream));
-  }
   HIPCHK(hipStreamSynchronize(b->stream));
-  if (b->parts > 1 && *b->h_abort != 0u) {
+  if (b->split && *b->h_abort != 0u) {
     *b->h_abort = 0u;
     return fail(REMD_EHIP, "resident batch launch aborted: the parts of a replica were not co-resident");
   }
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
   b->last_ms = ms;
+  if (!b->tuned) {
+    if (b->warm > 0) {
+      b->warm--;
+    } else {
+      b->cand_ms[b->cand_cur] = ms;
+      if (b->cand_cur + 1 < (int)b->cands.size()) {
+        b->cand_cur++;
+      } else {
+        b->tuned = true;
+        b->cand_cur = (int)(std::min_element(b->cand_ms.begin(), b->cand_ms.end()) - b->cand_ms.begin());
+      }
+      int rc = apply_plan(b, b->cands[b->cand_cur]);
+      if (rc) return rc;
+    }
+  }
   return 0;
 }
 
@@ -1189,14 +1433,15 @@ extern "C" void remd_batch_destroy(remd_batch *b) {
   if (b->stream) hipStreamSynchronize(b->stream);
   hipFree(b->d_table);
   hipFree(b->d_sync);
+  hipFree(b->d_blk);
   if (b->h_abort) hipHostFree(b->h_abort);
-  if (b->evres) hipEventDestroy(b->evres);
   for (int c = 0; c < NCLASS; c++) {
     if (b->cdone[c]) hipEventDestroy(b->cdone[c]);
     if (b->cstream[c]) hipStreamDestroy(b->cstream[c]);
   }
   if (b->ev0) hipEventDestroy(b->ev0);
   if (b->ev1) hipEventDestroy(b->ev1);
+  if (b->efork) hipEventDestroy(b->efork);
   if (b->stream) hipStreamDestroy(b->stream);
   delete b;
 }

@@ -1855,6 +1855,45 @@ __global__ __launch_bounds__(NT) void resident_kernel(LaunchArgs a) {
   run_resident<NT, NS, PT>(a, R, J, smem, g);
 }
 
+// One launch for every resident replica of a batch, whatever its water
+// model and pair precision.  Workgroup b runs the entry blk[b] = rep << 12 |
+// variant << 8 | parts << 4 | part (variant = 2 (nsite - 3) + prec), or
+// nothing for ~0u.  The host lists the parts of every split replica first,
+// the parts of one replica on block indices equal mod 8 (one XCD), all of
+// them within the workgroups the device holds at once; the single-part
+// replicas follow, heaviest first, and queue for workgroup slots as they
+// free up.  Workgroups are dispatched in index order, so every split
+// replica has all its parts resident before a queued replica starts, and
+// no part ever waits on a workgroup that is not yet running.
+__global__ __launch_bounds__(256) void resident_batch_kernel(LaunchArgs a) {
+  extern __shared__ __align__(16) char smem[];
+  const unsigned e = a.blk[blockIdx.x];
+  if (e == ~0u) return;
+  const int rep = (int)(e >> 12);
+  if (rep >= a.nrep) return;
+  const DescD R = ((const DescD *)a.reps)[rep];
+  JRef J = *R.job;
+  Group g;
+  const int parts = (int)((e >> 4) & 15u);
+  g.parts = parts < 1 ? 1 : parts;
+  g.part = (int)(e & 15u);
+  g.ctr = a.sync + rep;
+  g.abort = a.abort;
+  g.same_xcd = false;
+  g.mode = a.same_xcd_mode;
+  g.xbuf = gen(R.xbuf);
+  g.epoch = 0;
+  switch ((e >> 8) & 15u) {
+    case 0: run_resident<256, 3, float>(a, R, J, smem, g); break;
+    case 1: run_resident<256, 3, double>(a, R, J, smem, g); break;
+    case 2: run_resident<256, 4, float>(a, R, J, smem, g); break;
+    case 3: run_resident<256, 4, double>(a, R, J, smem, g); break;
+    case 4: run_resident<256, 5, float>(a, R, J, smem, g); break;
+    case 5: run_resident<256, 5, double>(a, R, J, smem, g); break;
+    default: break;
+  }
+}
+
 // ---------------------------------------------------------------- b1
 // ForceCalculator::CalcWallForce, tube (src/force_calculator.cxx:117-161):
 // one thread per table node, 10000-point phi sum in FP64 with float sqrt as
@@ -2068,6 +2107,17 @@ hipError_t launch_resident(const LaunchArgs &a, int nsite, int prec, int lds, hi
   return hipGetLastError();
 }
 
+hipError_t launch_resident_batch(const LaunchArgs &a, int lds, hipStream_t st) {
+  if (a.nblk <= 0 || !a.blk) return hipSuccess;
+  hipLaunchKernelGGL(resident_batch_kernel, dim3(a.nblk), dim3(256), lds, st, a);
+  return hipGetLastError();
+}
+
+hipError_t resident_batch_occupancy(int lds, int *per_cu) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, (const void *)resident_batch_kernel, 256,
+                                                      (size_t)lds);
+}
+
 // Workgroups of the resident kernel one CU holds at once with lds bytes.
 hipError_t resident_occupancy(int nsite, int prec, int lds, int *per_cu) {
   const void *f = resident_fn(nsite, prec);
@@ -2105,6 +2155,9 @@ hipError_t set_kernel_lds_limit(int bytes) {
   for (int ns = 3; ns <= 5 && e == hipSuccess; ns++)
     for (int pr = 0; pr < 2 && e == hipSuccess; pr++)
       e = hipFuncSetAttribute(resident_fn(ns, pr), hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void *)resident_batch_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            bytes);
   if (e == hipSuccess) e = set_lds_nt<256>(bytes);
   if (e == hipSuccess) e = set_lds_nt<512>(bytes);
   if (e == hipSuccess) e = set_lds_nt<128>(bytes);

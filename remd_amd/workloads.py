@@ -98,11 +98,13 @@ def _timed_intervals(run_one, n):
     return time.perf_counter() - t0, lat
 
 
-def uneven_batch_study(intervals=5, warmup=1):
+def uneven_batch_study(intervals=10, warmup=6):
     """Eight nanotube systems (R = 5..12 A, Lz = 40 A, nmol ~40-420,
     TIP3P/TIP4P alternating, 32 replicas each = 256 replicas): all jobs in
-    one batch (one launch per kernel class per interval) against the same
-    jobs one at a time (each job's replicas in one launch)."""
+    one batch (one resident launch per interval, replicas split over
+    workgroups by the batch's packing plan; the warm-up intervals include
+    the autotuner's timed candidates) against the same jobs one at a time
+    (each job's replicas in one launch)."""
     import numpy as np
 
     from . import Batch, GlibcRand, exchange_host
@@ -142,7 +144,8 @@ def uneven_batch_study(intervals=5, warmup=1):
         batched()
     wb, lat_b = _timed_intervals(batched, intervals)
     _progress(f"batched: {intervals} intervals in {wb:.3f} s")
-    parts = jobs[0].launch_path()
+    parts = bt.parts()
+    kms = bt.last_kernel_ms()
     bt.close()
     for _ in range(warmup):
         one_at_a_time()
@@ -151,7 +154,7 @@ def uneven_batch_study(intervals=5, warmup=1):
     rs = sum(s.nreplica * s.interval for s in systems) * intervals
     out = dict(batched=rs / wb, one_job_at_a_time=rs / ws, replicas=sum(s.nreplica for s in systems),
                nmol=[s.nmol for s in systems], models=[s.model for s in systems],
-               paths=[j.launch_path()[0] for j in jobs], parts_batched=parts[1],
+               paths=[j.launch_path()[0] for j in jobs], parts_batched=parts, batched_kernel_ms=kms,
                interval_latency_ms_batched={"p50": float(np.percentile(lat_b, 50)) * 1e3,
                                             "p99": float(np.percentile(lat_b, 99)) * 1e3})
     for j in jobs:

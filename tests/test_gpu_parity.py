@@ -223,14 +223,19 @@ def test_resident_path_agrees_with_global_path(prec, tol):
 
 
 def test_parts_do_not_change_results():
-    """A replica split over 7 workgroups (alone) and over 1 (batched with a
-    254-replica filler job that takes every slot) gives the same bits."""
+    """A replica split over 7 workgroups (alone) gives the same bits as in a
+    batch with a 254-replica filler job, over three intervals run with the
+    batch's plan fixed to 1 and to 3 workgroups per replica, then chosen by
+    the autotuner."""
     a = inp.config_tube(seed=31, nmol=60, Lz=26.0, ntemp=2, npress=1, nfwall=500)
     a.interval = 20
+    a.dt_fs = 0.5  # unrelaxed random placement: short steps keep it finite
     g = ra.REMDGPU(a)
-    g.execute_steps(a.T, a.P)
+    for _ in range(3):
+        g.execute_steps(a.T, a.P)
     parts_alone = g.launch_path()[1]
     ref = (g.all_averages(), g.state(0)["r"], g.state(1)["p"])
+    assert np.all(np.isfinite(ref[0][:, :15]))
     g.close()
     filler = inp.config_tube(seed=33, nmol=16, Lz=22.0, ntemp=127, npress=2, nfwall=300)
     filler.interval = 20
@@ -238,15 +243,20 @@ def test_parts_do_not_change_results():
     bt = ra.Batch()
     bt.add(ga)
     bt.add(gf)
-    ga.set_conditions(a.T, a.P)
-    gf.set_conditions(filler.T, filler.P)
-    bt.execute()
-    parts_batch = ga.launch_path()[1]
-    print("parts alone", parts_alone, "batched", parts_batch)
-    assert parts_alone > 1 and parts_batch == 1
+    seen = []
+    for plan in ([1, 1], [3, 1], None):
+        bt.set_parts(plan)
+        ga.set_conditions(a.T, a.P)
+        gf.set_conditions(filler.T, filler.P)
+        bt.execute()
+        seen.append(bt.parts())
+    print("parts alone", parts_alone, "batched", seen)
+    assert parts_alone > 1 and seen[0] == [1, 1] and seen[1] == [3, 1]
     assert np.array_equal(ga.all_averages(), ref[0])
     assert np.array_equal(ga.state(0)["r"], ref[1])
     assert np.array_equal(ga.state(1)["p"], ref[2])
+    with pytest.raises(RuntimeError):
+        bt.set_parts([9, 1])
     bt.close()
     ga.close()
     gf.close()
@@ -278,6 +288,49 @@ def test_mixed_path_batch_equals_alone():
     bt.close()
     ga.close()
     gb.close()
+
+
+def test_uneven_autotuned_batch_equals_alone():
+    """Four jobs of different sizes, models, confinements and pair
+    precisions in one batch for six intervals -- the autotuner times
+    different packing plans on the first ones -- give the bits each gives
+    alone."""
+    from remd_amd.workloads import relax
+
+    jobs = [inp.config_tube(seed=51, nmol=40, Lz=22.0, ntemp=4, npress=1, model="TIP3P", nfwall=400),
+            inp.config_tube(seed=52, nmol=90, Lz=30.0, ntemp=4, npress=2, nfwall=400),
+            inp.config_slit(seed=53, nmol=120, ntemp=2, npress=2, model="TIP5P", Lxy=30.0),
+            inp.config_bulk(seed=54, nmol=64, ntemp=2, model="TIP4P")]
+    precs = [0, 0, 0, 1]
+    jobs = [relax(s) for s in jobs]  # random placements, equilibrated at 0.5 fs
+    for s in jobs:
+        s.interval = 20
+    alone = []
+    for s, pr in zip(jobs, precs):
+        g = ra.REMDGPU(s, precision=pr)
+        for _ in range(6):
+            g.execute_steps(s.T, s.P)
+        alone.append((g.all_averages(), g.state(0)["r"], g.state(s.nreplica - 1)["p"]))
+        assert np.all(np.isfinite(alone[-1][0][:, :15]))
+        g.close()
+    gs = [ra.REMDGPU(s, precision=pr) for s, pr in zip(jobs, precs)]
+    bt = ra.Batch()
+    for g in gs:
+        bt.add(g)
+    plans = []
+    for _ in range(6):
+        for g, s in zip(gs, jobs):
+            g.set_conditions(s.T, s.P)
+        bt.execute()
+        plans.append(tuple(bt.parts()))
+    print("plans", plans)
+    for g, s, ref in zip(gs, jobs, alone):
+        assert np.array_equal(g.all_averages(), ref[0])
+        assert np.array_equal(g.state(0)["r"], ref[1])
+        assert np.array_equal(g.state(s.nreplica - 1)["p"], ref[2])
+    bt.close()
+    for g in gs:
+        g.close()
 
 
 @pytest.mark.parametrize("path", PATHS)
