@@ -70,6 +70,7 @@ struct JobParams {
   double wall_length;           // A
   double bottom_area;           // pi (R - 1.7)^2 for the tube
   double mass, inertia[3];
+  double inv_mass, inv_inertia[3]; // 1 / mass, 1 / inertia (host-computed)
   double site_r[MAXSITE][3];    // body-frame site coordinates
   double q[MAXSITE];            // reduced charges
   double lj_sig2, lj_eps4;      // site 0 LJ: sigma^2, 4 epsilon

@@ -269,6 +269,8 @@ static void fill_params(const remd_gpu_desc *d, JobParams &J) {
   for (int a = 0; a < d->nsite; a++) m += d->site_m[a];
   J.mass = d->mass > 0 ? d->mass : m;
   for (int k = 0; k < 3; k++) J.inertia[k] = d->inertia[k];
+  J.inv_mass = 1.0 / J.mass;
+  for (int k = 0; k < 3; k++) J.inv_inertia[k] = J.inertia[k] != 0.0 ? 1.0 / J.inertia[k] : 0.0;
   for (int a = 0; a < d->nsite; a++) {
     for (int k = 0; k < 3; k++) J.site_r[a][k] = d->site_r[a][k];
     J.q[a] = d->site_q[a];

@@ -113,11 +113,31 @@ __device__ inline Lds lds_carve(char *smem, int nmol, int nsite, int prec) {
 }
 
 // ------------------------------------------------------------ reductions
+// Lane exchange inside rows of 16 lanes (DPP control CTRL) of a double.
+template <int CTRL>
+__device__ inline double dpp_mov(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes of a full wave; every lane receives the same bits.
+// Inside each row of 16 lanes: quad_perm swaps (lane ^ 1, lane ^ 2), then
+// the half-row mirror (lane 7 - l) and the row mirror (lane 15 - l); each
+// step adds the same two operands as the partner lane (a + b == b + a
+// exactly), so a row's lanes agree bit for bit.  The four row sums are read
+// from lanes 0, 16, 32 and 48 and added in a fixed order.
 __device__ inline double wave_sum(double v) {
-  // xor butterfly: a+b == b+a exactly, so every lane ends with the same bits
-#pragma unroll
-  for (int m = 1; m < WAVE; m <<= 1) v += __shfl_xor(v, m, WAVE);
-  return v;
+  v += dpp_mov<0xB1>(v);   // quad_perm [1, 0, 3, 2]
+  v += dpp_mov<0x4E>(v);   // quad_perm [2, 3, 0, 1]
+  v += dpp_mov<0x141>(v);  // row_half_mirror
+  v += dpp_mov<0x140>(v);  // row_mirror
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  const double r0 = __hiloint2double(__builtin_amdgcn_readlane(hi, 0), __builtin_amdgcn_readlane(lo, 0));
+  const double r1 = __hiloint2double(__builtin_amdgcn_readlane(hi, 16), __builtin_amdgcn_readlane(lo, 16));
+  const double r2 = __hiloint2double(__builtin_amdgcn_readlane(hi, 32), __builtin_amdgcn_readlane(lo, 32));
+  const double r3 = __hiloint2double(__builtin_amdgcn_readlane(hi, 48), __builtin_amdgcn_readlane(lo, 48));
+  return (r0 + r1) + (r2 + r3);
 }
 
 // Sum K doubles over the workgroup; every thread receives the same result.
@@ -211,7 +231,7 @@ __device__ inline void step_sincos(double x, double *sn, double *cs) {
 
 // Free rotation about body axis k (src/integrator.cxx:467-563, P1..P3 of
 // src/integrator.h:321-346).  Returns xi.
-__device__ inline double rotate(double *q, double *p, int axis, double coef_xi, double I, double h) {
+__device__ inline double rotate(double *q, double *p, int axis, double coef_xi, double invI, double h) {
   double Pq[4], Pp[4];
   if (axis == 0) {
     Pq[0] = -q[1]; Pq[1] = q[0]; Pq[2] = q[3]; Pq[3] = -q[2];
@@ -223,7 +243,7 @@ __device__ inline double rotate(double *q, double *p, int axis, double coef_xi, 
     Pq[0] = -q[3]; Pq[1] = q[2]; Pq[2] = -q[1]; Pq[3] = q[0];
     Pp[0] = -p[3]; Pp[1] = p[2]; Pp[2] = -p[1]; Pp[3] = p[0];
   }
-  const double xi = (p[0] * Pq[0] + p[1] * Pq[1] + p[2] * Pq[2] + p[3] * Pq[3]) * coef_xi / I;
+  const double xi = (p[0] * Pq[0] + p[1] * Pq[1] + p[2] * Pq[2] + p[3] * Pq[3]) * coef_xi * invI;
   double sn, c;
   step_sincos(xi * h, &sn, &c);
 #pragma unroll
@@ -289,10 +309,10 @@ __device__ inline void D4(JRef J, Sc &c) {
 // p += S(q) n 2 s dt/2.
 __device__ inline void D5_mol(JRef J, const Sc &c, double *v, const double *f,
                               double *p, const double *q, const double *n) {
-  const double m = J.mass;
-  v[0] += f[0] * (c.L[0] * c.s * J.dthalf) / m;
-  v[1] += f[1] * (c.L[1] * c.s * J.dthalf) / m;
-  v[2] += f[2] * (c.L[2] * c.s * J.dthalf) / m;
+  const double cm = c.s * J.dthalf * J.inv_mass;
+  v[0] += f[0] * (c.L[0] * cm);
+  v[1] += f[1] * (c.L[1] * cm);
+  v[2] += f[2] * (c.L[2] * cm);
   const double cp = 2.0 * c.s * J.dthalf;
   const double d0 = q[0] * n[0] - q[1] * n[1] - q[2] * n[2] - q[3] * n[3];
   const double d1 = q[1] * n[0] + q[0] * n[1] - q[3] * n[2] + q[2] * n[3];
@@ -1031,6 +1051,7 @@ __device__ void first_half(JRef J, double *mol, int N, Sc &c, double *red) {
       coef_r[d] = Li * Li * si * J.dt;
     }
     const double *I = (const double *)J.inertia;
+    const double *iI = (const double *)J.inv_inertia;
     for (int i = threadIdx.x; i < N; i += NT) {
       double r[3] = {mol[F_RX * N + i], mol[F_RY * N + i], mol[F_RZ * N + i]};
       double v[3] = {mol[F_VX * N + i], mol[F_VY * N + i], mol[F_VZ * N + i]};
@@ -1040,9 +1061,9 @@ __device__ void first_half(JRef J, double *mol, int N, Sc &c, double *red) {
       const double n[4] = {mol[F_N0 * N + i], mol[F_N1 * N + i], mol[F_N2 * N + i], mol[F_N3 * N + i]};
       D5_mol(J, c_old, v, f, p, q, n);
       double xi;
-      xi = rotate(q, p, 2, coef_xi, I[2], J.dthalf);
+      xi = rotate(q, p, 2, coef_xi, iI[2], J.dthalf);
       if (T1) sums[3] += 0.5 * xi * xi * I[2];
-      xi = rotate(q, p, 1, coef_xi, I[1], J.dthalf);
+      xi = rotate(q, p, 1, coef_xi, iI[1], J.dthalf);
       if (T1) sums[3] += 0.5 * xi * xi * I[1];
 #pragma unroll
       for (int d = 0; d < 3; d++) {
@@ -1050,16 +1071,16 @@ __device__ void first_half(JRef J, double *mol, int N, Sc &c, double *red) {
         if (r[d] < 0.0) r[d] += 1.0;
         if (r[d] >= 1.0) r[d] -= 1.0;
       }
-      xi = rotate(q, p, 0, 0.25 * si, I[0], J.dt);
+      xi = rotate(q, p, 0, 0.25 * si, iI[0], J.dt);
       if (needPv) {
         sums[0] += 2.0 * J.mass * v[0] * v[0];
         sums[1] += 2.0 * J.mass * v[1] * v[1];
         sums[2] += 2.0 * J.mass * v[2] * v[2];
       }
       if (T1) sums[3] += xi * xi * I[0];
-      xi = rotate(q, p, 1, coef_xi, I[1], J.dthalf);
+      xi = rotate(q, p, 1, coef_xi, iI[1], J.dthalf);
       if (T1) sums[3] += 0.5 * xi * xi * I[1];
-      xi = rotate(q, p, 2, coef_xi, I[2], J.dthalf);
+      xi = rotate(q, p, 2, coef_xi, iI[2], J.dthalf);
       if (T1) sums[3] += 0.5 * xi * xi * I[2];
       mol[F_RX * N + i] = r[0]; mol[F_RY * N + i] = r[1]; mol[F_RZ * N + i] = r[2];
       mol[F_VX * N + i] = v[0]; mol[F_VY * N + i] = v[1]; mol[F_VZ * N + i] = v[2];
@@ -1364,33 +1385,55 @@ __device__ bool group_wait(Group &g, int *flag, double *pt_sc = nullptr, bool ce
 // stable sort on the double keys: rank = #smaller + #equal-and-earlier),
 // permutation of the state, then the windows (src/molecules.cxx:541-585
 // semantics; raw js/je and the whole-ring flag go to win_dbg when given).
+// One pass over the keys per molecule gives its rank and the four key
+// counts the window scan needs (#keys < z - w, < z - w + 1, <= z + w,
+// <= z + w - 1: the scan's bounds do not depend on the order of the keys).
 template <int NT, int NS>
 __device__ void rank_sort_resident(JRef J, double *mol, int *mid, int N, double Laxis,
                                    double margin, const LdsR &S, int *win_dbg) {
   double *z = (double *)S.region0;
   uint16_t *rk = (uint16_t *)(S.region0 + 8 * N);
-  double *tmp = (double *)(S.region0 + align16(10 * N));
+  // window counts, indexed by sorted position until the windows are built
+  // (the per-molecule energy rows are idle during the sort)
+  uint16_t *klo = S.js, *khi = S.je;
+  uint16_t *klo1 = (uint16_t *)S.E, *khi1 = klo1 + N;
   double *ax = mol + (F_RX + J.sort_axis) * N;
   for (int i = threadIdx.x; i < N; i += NT) z[i] = ax[i] + 0.0;  // -0.0 -> +0.0
   __syncthreads();
+  const double w = (J.rcut + margin) / Laxis;
   for (int i = threadIdx.x; i < N; i += NT) {
     const double zi = z[i];
-    int r = 0;
+    const double lb = zi - w, lb1 = (zi - w) + 1.0;
+    const double ub = zi + w, ub1 = (zi + w) - 1.0;
+    int r = 0, a = 0, a1 = 0, b = 0, b1 = 0;
+#pragma unroll 8
     for (int j = 0; j < N; ++j) {
       const double zj = z[j];
-      r += (zj < zi || (zj == zi && j < i)) ? 1 : 0;
+      r += (j < i) ? (zj <= zi ? 1 : 0) : (zj < zi ? 1 : 0);
+      a += zj < lb ? 1 : 0;
+      a1 += zj < lb1 ? 1 : 0;
+      b += zj <= ub ? 1 : 0;
+      b1 += zj <= ub1 ? 1 : 0;
     }
     rk[i] = (uint16_t)r;
+    klo[r] = (uint16_t)a;
+    klo1[r] = (uint16_t)a1;
+    khi[r] = (uint16_t)b;
+    khi1[r] = (uint16_t)b1;
   }
   __syncthreads();
-  // permute the state through region 0, as many fields per round as fit
-  const int fit = (resident_region0(N, NS, J.precision) - align16(10 * N)) / (8 * N);
+  // permute the state through the larger of the pair-partial area (idle
+  // during the sort) and the rest of region 0, as many fields per round as fit
+  const int cap_part = (int)((S.E - S.part) / N);  // S.part .. S.E is the partial area
+  const int cap_r0 = (resident_region0(N, NS, J.precision) - align16(10 * N)) / (8 * N);
+  double *tmp = cap_part >= cap_r0 ? S.part : (double *)(S.region0 + align16(10 * N));
+  const int fit = cap_part >= cap_r0 ? cap_part : cap_r0;
   const int per = fit < 1 ? 1 : (fit > NFIELD ? NFIELD : fit);
   for (int f0 = 0; f0 < NFIELD; f0 += per) {
     const int nf = (f0 + per <= NFIELD) ? per : NFIELD - f0;
-    for (int k = threadIdx.x; k < nf * N; k += NT) {
-      const int f = k / N, i = k - f * N;
-      tmp[f * N + rk[i]] = mol[(f0 + f) * N + i];
+    for (int i = threadIdx.x; i < N; i += NT) {
+      const int ri = rk[i];
+      for (int f = 0; f < nf; ++f) tmp[f * N + ri] = mol[(f0 + f) * N + i];
     }
     __syncthreads();
     for (int k = threadIdx.x; k < nf * N; k += NT) mol[f0 * N + k] = tmp[k];
@@ -1403,10 +1446,14 @@ __device__ void rank_sort_resident(JRef J, double *mol, int *mid, int N, double 
     for (int i = threadIdx.x; i < N; i += NT) mid[i] = ti[i];
     __syncthreads();
   }
-  const double w = (J.rcut + margin) / Laxis;
+  // windows (window_of's result from the counts; window_full as the scan)
   for (int i = threadIdx.x; i < N; i += NT) {
+    const int a = klo[i], b = khi[i];
     int js, je;
-    window_of(ax, N, i, w, js, je);
+    if (a >= 1) js = a - 1;
+    else js = (klo1[i] >= 1) ? klo1[i] - 1 : N - 1;
+    if (b < N) je = b;
+    else je = (khi1[i] < N) ? khi1[i] : 0;
     const bool full = window_full(ax, i, w, js, je);
     if (win_dbg) {
       win_dbg[3 * i] = js;
