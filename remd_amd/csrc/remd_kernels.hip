@@ -1383,43 +1383,26 @@ __device__ bool group_wait(Group &g, int *flag, double *pt_sc = nullptr, bool ce
 
 // Stable rank sort of the sort-axis coordinate in LDS (the same order as a
 // stable sort on the double keys: rank = #smaller + #equal-and-earlier),
-// permutation of the state, then the windows (src/molecules.cxx:541-585
-// semantics; raw js/je and the whole-ring flag go to win_dbg when given).
-// One pass over the keys per molecule gives its rank and the four key
-// counts the window scan needs (#keys < z - w, < z - w + 1, <= z + w,
-// <= z + w - 1: the scan's bounds do not depend on the order of the keys).
+// permutation of the state, then the windows by binary search on the sorted
+// keys (src/molecules.cxx:541-585 semantics; raw js/je and the whole-ring
+// flag go to win_dbg when given).
 template <int NT, int NS>
 __device__ void rank_sort_resident(JRef J, double *mol, int *mid, int N, double Laxis,
                                    double margin, const LdsR &S, int *win_dbg) {
   double *z = (double *)S.region0;
   uint16_t *rk = (uint16_t *)(S.region0 + 8 * N);
-  // window counts, indexed by sorted position until the windows are built
-  // (the per-molecule energy rows are idle during the sort)
-  uint16_t *klo = S.js, *khi = S.je;
-  uint16_t *klo1 = (uint16_t *)S.E, *khi1 = klo1 + N;
   double *ax = mol + (F_RX + J.sort_axis) * N;
   for (int i = threadIdx.x; i < N; i += NT) z[i] = ax[i] + 0.0;  // -0.0 -> +0.0
   __syncthreads();
-  const double w = (J.rcut + margin) / Laxis;
   for (int i = threadIdx.x; i < N; i += NT) {
     const double zi = z[i];
-    const double lb = zi - w, lb1 = (zi - w) + 1.0;
-    const double ub = zi + w, ub1 = (zi + w) - 1.0;
-    int r = 0, a = 0, a1 = 0, b = 0, b1 = 0;
+    int r = 0;
 #pragma unroll 8
     for (int j = 0; j < N; ++j) {
       const double zj = z[j];
       r += (j < i) ? (zj <= zi ? 1 : 0) : (zj < zi ? 1 : 0);
-      a += zj < lb ? 1 : 0;
-      a1 += zj < lb1 ? 1 : 0;
-      b += zj <= ub ? 1 : 0;
-      b1 += zj <= ub1 ? 1 : 0;
     }
     rk[i] = (uint16_t)r;
-    klo[r] = (uint16_t)a;
-    klo1[r] = (uint16_t)a1;
-    khi[r] = (uint16_t)b;
-    khi1[r] = (uint16_t)b1;
   }
   __syncthreads();
   // permute the state through the larger of the pair-partial area (idle
@@ -1446,14 +1429,10 @@ __device__ void rank_sort_resident(JRef J, double *mol, int *mid, int N, double 
     for (int i = threadIdx.x; i < N; i += NT) mid[i] = ti[i];
     __syncthreads();
   }
-  // windows (window_of's result from the counts; window_full as the scan)
+  const double w = (J.rcut + margin) / Laxis;
   for (int i = threadIdx.x; i < N; i += NT) {
-    const int a = klo[i], b = khi[i];
     int js, je;
-    if (a >= 1) js = a - 1;
-    else js = (klo1[i] >= 1) ? klo1[i] - 1 : N - 1;
-    if (b < N) je = b;
-    else je = (khi1[i] < N) ? khi1[i] : 0;
+    window_of(ax, N, i, w, js, je);
     const bool full = window_full(ax, i, w, js, je);
     if (win_dbg) {
       win_dbg[3 * i] = js;
