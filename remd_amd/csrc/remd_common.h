@@ -93,6 +93,10 @@ struct ReplicaDesc {
   double *xbuf;                 // resident path with parts > 1: 2 x nmol x 16 exchange rows
 };
 
+// Resident path: molecules per energy block.  A replica's parts own whole
+// blocks, and the energy/virial totals are sums of block sums.
+constexpr int EB = 4;
+
 struct LaunchArgs {
   const ReplicaDesc *reps;
   int nrep;

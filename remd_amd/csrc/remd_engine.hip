@@ -489,7 +489,8 @@ extern "C" int remd_gpu_create(remd_gpu **out, const remd_gpu_desc *d, const rem
     // the part count this job gets alone; fixed for the job's lifetime, so
     // results do not depend on the part count of a later launch
     const int gn = pick_parts(g->capacity, g->nrep, g->N);
-    const int M = (g->N + gn - 1) / gn;
+    const int nb = (g->N + EB - 1) / EB;
+    const int M = ((nb + gn - 1) / gn) * EB;  // molecules of the largest part
     const int wcount = std::max(1, (int)std::ceil(wf * g->N));
     g->prm.chunks = d->chunks ? d->chunks : std::max(1, std::min(std::min(64, 256 / M), wcount));
   } else {

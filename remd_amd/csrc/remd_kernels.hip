@@ -1243,7 +1243,8 @@ __global__ __launch_bounds__(NT) void interval_kernel(LaunchArgs a) {
 // energies are reduced from per-molecule values in a fixed order, so results
 // do not depend on G, on batching or on placement.
 
-constexpr int XW = 16;                 // doubles per molecule row of the force exchange
+constexpr int XW = 16;                 // doubles per molecule of the exchange buffer (per parity)
+constexpr int XR = 8;                  // doubles per molecule row: F[3], n[4]
 constexpr unsigned MAXSPIN = 1u << 20; // group-barrier polls before the launch gives up
 
 // Per-item partials of the pair pass: COM force (3), quaternion torque (4),
@@ -1258,7 +1259,7 @@ struct LdsR {
   char *region0;    // site records, or the sort scratch
   uint16_t *js, *je;
   double *part;     // NV x NT pair partials, value-major
-  double *E;        // 6 x N per-molecule lj, clmb, wall, vir[3]
+  double *E;        // energy-block sums lj, clmb, wall, vir[3] (6 x nblk; sized 6 x N)
   double *red;      // block_sum scratch
   int *flag;        // barrier status broadcast (16 ints)
   double *ptimer;   // 16 phase-timer accumulators (REMD_PHASE_TIMERS builds)
@@ -1472,12 +1473,15 @@ __device__ void window_from_debug(const int *win, int N, const LdsR &S) {
 // src/force_calculator.cxx:803-989 semantics; wall src/force_calculator.cxx:
 // 1494-1531; A2M torque src/molecules.cxx:401-439).  Work item (i, c): chunk
 // c of C runs over i's window with stride C; the C partials of i are summed
-// in chunk order by one thread.  Outputs per molecule: F[3], n[4], lj, clmb,
-// wall, vir[3] -- into xrow (global exchange row) when given, else into the
-// LDS state and S.E.
+// in chunk order by one thread.  Outputs per molecule: F[3], n[4] into the
+// LDS state (and into the global exchange row xrow when given); per energy
+// block of EB molecules: the lj, clmb, wall, vir[3] sums into Bk (6 x nblk,
+// value-major) and into the global block row xblk when given.  i0 is a
+// block boundary.
 template <int NT, int NS, typename PT>
 __device__ void force_range(JRef J, double *mol, int N, const Sc &c, const LdsR &S, int i0,
-                            int i1, double *xrow, double *site_dbg, double *pt_sc = nullptr) {
+                            int i1, double *xrow, double *Bk, int nblk, double *xblk, double *site_dbg,
+                            double *pt_sc = nullptr) {
   PT_DECL
   (void)pt_sc;
   constexpr int QB = (NS == 3) ? 0 : 1;
@@ -1522,7 +1526,7 @@ __device__ void force_range(JRef J, double *mol, int N, const Sc &c, const LdsR 
     k.per[d] = J.per[d];
   }
   const int C = J.chunks;
-  const int B = NT / C;  // molecules per round
+  const int B = (NT / C) / EB * EB;  // molecules per round, whole energy blocks
   const int NV = npart(NS, site_dbg != nullptr);
   const double dr = (J.cmode == 1) ? J.wall_length / (double)J.nfwall : 0.5 * J.wall_length / (double)J.nfwall;
   const double dri = 1.0 / dr;
@@ -1639,12 +1643,15 @@ __device__ void force_range(JRef J, double *mol, int N, const Sc &c, const LdsR 
     }
     __syncthreads();
     PT_MARK(12, pt_sc);
-    if (t < ie - ib) {
+    const bool mine = t < ie - ib;
+    double v[NPART];
+#pragma unroll
+    for (int u = 0; u < NPART; u++) v[u] = 0.0;
+    if (mine) {
       const int i = ib + t;
       const double *P = S.part + t * C;
       // chunks in order for every value; chunk-outer so each chunk's loads
       // issue together
-      double v[NPART];
 #pragma unroll
       for (int u = 0; u < NPART; u++) v[u] = P[u * NT];
       for (int cc = 1; cc < C; cc++) {
@@ -1654,30 +1661,47 @@ __device__ void force_range(JRef J, double *mol, int N, const Sc &c, const LdsR 
 #pragma unroll
         for (int u = 0; u < NPART; u++) v[u] += w[u];
       }
-      // F, n, then lj, clmb, wall, vir
-      const double ev[6] = {v[10], v[11], v[12], v[7], v[8], v[9]};
+      // F, n into the state (and the exchange row)
       if (xrow) {
-        double *o = xrow + (size_t)i * XW;
+        double *o = xrow + (size_t)i * XR;
 #pragma unroll
         for (int u = 0; u < 7; u++) o[u] = v[u];
-#pragma unroll
-        for (int u = 0; u < 6; u++) o[7 + u] = ev[u];
-      } else {
-        mol[F_FX * N + i] = v[0];
-        mol[F_FY * N + i] = v[1];
-        mol[F_FZ * N + i] = v[2];
-        mol[F_N0 * N + i] = v[3];
-        mol[F_N1 * N + i] = v[4];
-        mol[F_N2 * N + i] = v[5];
-        mol[F_N3 * N + i] = v[6];
-#pragma unroll
-        for (int u = 0; u < 6; u++) S.E[u * N + i] = ev[u];
       }
+      mol[F_FX * N + i] = v[0];
+      mol[F_FY * N + i] = v[1];
+      mol[F_FZ * N + i] = v[2];
+      mol[F_N0 * N + i] = v[3];
+      mol[F_N1 * N + i] = v[4];
+      mol[F_N2 * N + i] = v[5];
+      mol[F_N3 * N + i] = v[6];
       if (site_dbg) {
         for (int u = NPART; u < NV; u++) {
           double sum = P[u * NT];
           for (int cc = 1; cc < C; cc++) sum += P[u * NT + cc];
           site_dbg[i * NS * 3 + (u - NPART)] = sum;
+        }
+      }
+    }
+    // energy-block sums: the EB = 4 molecules of a block sit on one lane
+    // quad (rounds start on block boundaries); lane 4k adds them in
+    // molecule order (absent molecules are zeros)
+    {
+      const double ev[6] = {v[10], v[11], v[12], v[7], v[8], v[9]};
+      double bs[6];
+#pragma unroll
+      for (int u = 0; u < 6; u++) {
+        const double x1 = dpp_mov<0x55>(ev[u]);  // quad_perm [1, 1, 1, 1]
+        const double x2 = dpp_mov<0xAA>(ev[u]);  // quad_perm [2, 2, 2, 2]
+        const double x3 = dpp_mov<0xFF>(ev[u]);  // quad_perm [3, 3, 3, 3]
+        bs[u] = ((ev[u] + x1) + x2) + x3;
+      }
+      if (mine && (t & 3) == 0) {
+        const int b = (ib + t) / EB;
+#pragma unroll
+        for (int u = 0; u < 6; u++) Bk[u * nblk + b] = bs[u];
+        if (xblk) {
+#pragma unroll
+          for (int u = 0; u < 6; u++) xblk[(size_t)b * 8 + u] = bs[u];
         }
       }
     }
@@ -1687,8 +1711,11 @@ __device__ void force_range(JRef J, double *mol, int N, const Sc &c, const LdsR 
 }
 
 // Forces of the whole replica into the LDS state and the energy/virial
-// totals into c: this part's range, the exchange with the other parts,
-// then the fixed-order reduction of the per-molecule values.
+// totals into c: this part's molecules [i0, i1) (whole energy blocks), the
+// exchange with the other parts -- F and n per molecule, the six energy and
+// virial sums per block of EB molecules -- then the fixed-order reduction of
+// the block sums.  A block is summed in molecule order by one thread and
+// the blocks by a fixed tree, so the totals do not depend on the parts.
 template <int NT, int NS, typename PT>
 __device__ bool forces_resident(JRef J, double *mol, int N, Sc &c, const LdsR &S, Group &g,
                                 int i0, int i1, double *site_dbg, double *pt_sc = nullptr) {
@@ -1697,43 +1724,58 @@ __device__ bool forces_resident(JRef J, double *mol, int N, Sc &c, const LdsR &S
   m2a<NT, NS, PT>(J, mol, N, c.L[0], c.L[1], c.L[2], (PT *)S.region0);
   __syncthreads();
   PT_MARK(4, pt_sc);
+  const int nblk = (N + EB - 1) / EB;
+  const int b0 = i0 / EB, b1 = (i1 + EB - 1) / EB;
+  double *xrow = g.parts > 1 ? g.xbuf + (size_t)(g.epoch & 1u) * N * XW : nullptr;
+  double *xblk = xrow ? xrow + (size_t)N * XR : nullptr;
+  double *Bk = S.E;  // 6 x nblk block sums
+  force_range<NT, NS, PT>(J, mol, N, c, S, i0, i1, xrow, Bk, nblk, xblk, site_dbg, pt_sc);
+  PT_MARK(5, pt_sc);
   if (g.parts > 1) {
-    double *xrow = g.xbuf + (size_t)(g.epoch & 1u) * N * XW;
-    force_range<NT, NS, PT>(J, mol, N, c, S, i0, i1, xrow, site_dbg, pt_sc);
-    PT_MARK(5, pt_sc);
     if (!group_wait<NT>(g, S.flag, pt_sc)) return false;
     PT_MARK(6, pt_sc);
     const bool direct = g.same_xcd && !(g.mode & 2);
-    for (int i = threadIdx.x; i < N; i += NT) {
-      double *o = xrow + (size_t)i * XW;
-      double v[13];
+    // the other parts' rows and block sums, all loads of a thread issued
+    // together
+    const int nrow = N > nblk ? N : nblk;
+    for (int k = threadIdx.x; k < nrow; k += NT) {
+      const bool row = k < N && !(k >= i0 && k < i1);
+      const bool blk = k < nblk && !(k >= b0 && k < b1);
+      const double *o = xrow + (size_t)(row ? k : 0) * XR;
+      const double *ob = xblk + (size_t)(blk ? k : 0) * 8;
+      double v[7], e[6];
       if (direct) {
 #pragma unroll
-        for (int u = 0; u < 13; u++) v[u] = __hip_atomic_load(o + u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int u = 0; u < 7; u++) v[u] = row ? __hip_atomic_load(o + u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
+#pragma unroll
+        for (int u = 0; u < 6; u++) e[u] = blk ? __hip_atomic_load(ob + u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
       } else {
 #pragma unroll
-        for (int u = 0; u < 13; u++) v[u] = o[u];
+        for (int u = 0; u < 7; u++) v[u] = row ? o[u] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 6; u++) e[u] = blk ? ob[u] : 0.0;
       }
-      mol[F_FX * N + i] = v[0];
-      mol[F_FY * N + i] = v[1];
-      mol[F_FZ * N + i] = v[2];
-      mol[F_N0 * N + i] = v[3];
-      mol[F_N1 * N + i] = v[4];
-      mol[F_N2 * N + i] = v[5];
-      mol[F_N3 * N + i] = v[6];
+      if (row) {
+        mol[F_FX * N + k] = v[0];
+        mol[F_FY * N + k] = v[1];
+        mol[F_FZ * N + k] = v[2];
+        mol[F_N0 * N + k] = v[3];
+        mol[F_N1 * N + k] = v[4];
+        mol[F_N2 * N + k] = v[5];
+        mol[F_N3 * N + k] = v[6];
+      }
+      if (blk) {
 #pragma unroll
-      for (int u = 0; u < 6; u++) S.E[u * N + i] = v[7 + u];
+        for (int u = 0; u < 6; u++) Bk[u * nblk + k] = e[u];
+      }
     }
-    __syncthreads();
     PT_MARK(11, pt_sc);
-  } else {
-    force_range<NT, NS, PT>(J, mol, N, c, S, 0, N, nullptr, site_dbg);
-    PT_MARK(5, pt_sc);
   }
+  __syncthreads();
   double tot[6] = {0, 0, 0, 0, 0, 0};
-  for (int i = threadIdx.x; i < N; i += NT) {
+  for (int b = threadIdx.x; b < nblk; b += NT) {
 #pragma unroll
-    for (int u = 0; u < 6; u++) tot[u] += S.E[u * N + i];
+    for (int u = 0; u < 6; u++) tot[u] += Bk[u * nblk + b];
   }
   block_sum<NT, 6>(tot, S.red);
   c.lj = 0.5 * tot[0];
@@ -1772,8 +1814,11 @@ __device__ void run_resident(const LaunchArgs &a, const DescD &R, JRef J, char *
     g.same_xcd = a.same_xcd_ok && S.flag[1] != 0;
     __syncthreads();
   }
-  const int i0 = (int)(((long long)g.part * N) / g.parts);
-  const int i1 = (int)(((long long)(g.part + 1) * N) / g.parts);
+  // this part's molecules: whole energy blocks
+  const int nblk = (N + EB - 1) / EB;
+  const int i0 = (int)(((long long)g.part * nblk) / g.parts) * EB;
+  const int i1e = (int)(((long long)(g.part + 1) * nblk) / g.parts) * EB;
+  const int i1 = i1e < N ? i1e : N;
   double *site_dbg = gen(R.site_f);
   int *win_dbg = lead ? gen(R.win) : nullptr;
   double *sc_out = lead ? gen(R.sc) : nullptr;
