@@ -141,7 +141,9 @@ __device__ inline double wave_sum(double v) {
 }
 
 // Sum K doubles over the workgroup; every thread receives the same result.
-template <int NT, int K>
+// With SYNC_AFTER = false the caller guarantees a workgroup barrier before
+// red is written again.
+template <int NT, int K, bool SYNC_AFTER = true>
 __device__ inline void block_sum(double (&v)[K], double *red) {
   constexpr int NW = NT / WAVE;
   const int lane = threadIdx.x & (WAVE - 1);
@@ -159,18 +161,21 @@ __device__ inline void block_sum(double (&v)[K], double *red) {
     for (int ww = 0; ww < NW; ww++) s += red[ww * K + k];
     v[k] = s;
   }
-  __syncthreads();
+  if (SYNC_AFTER) __syncthreads();
 }
 
 // ------------------------------------------------------- replica scalars
 struct Sc {
   double s, Ps, Q, Pv[3], W, L[3], T, P, gkT, H0;
   double lj, clmb, wall, pot, vir[3];
+  double iQ, iW;  // 1 / Q, 1 / W
 };
 
 __device__ inline void load_sc(Sc &c, const double *sc) {
   c.s = sc[S_S]; c.Ps = sc[S_PS]; c.Q = sc[S_Q];
   c.Pv[0] = sc[S_PV0]; c.Pv[1] = sc[S_PV1]; c.Pv[2] = sc[S_PV2]; c.W = sc[S_W];
+  c.iQ = 1.0 / c.Q;
+  c.iW = 1.0 / c.W;
   c.L[0] = sc[S_L0]; c.L[1] = sc[S_L1]; c.L[2] = sc[S_L2];
   c.T = sc[S_T]; c.P = sc[S_P]; c.gkT = sc[S_GKT]; c.H0 = sc[S_H0];
   c.lj = sc[S_LJ]; c.clmb = sc[S_CLMB]; c.wall = sc[S_WALL]; c.pot = sc[S_POT];
@@ -262,7 +267,7 @@ __device__ inline int teff(JRef J) { return J.tmode == 1 ? 1 : 0; }
 // D6 (src/integrator.cxx:634-644)
 __device__ inline void D6(JRef J, Sc &c) {
   if (teff(J)) {
-    const double tmp = 1.0 + (c.Ps / (2.0 * c.Q) * J.dthalf);
+    const double tmp = 1.0 + (c.Ps * (0.5 * c.iQ) * J.dthalf);
     c.s *= tmp * tmp;
     c.Ps /= tmp;
   }
@@ -288,18 +293,18 @@ __device__ inline void D5_scalars(JRef J, Sc &c) {
 // term of remdgpu.cu:2090-2095)
 __device__ inline void D4(JRef J, Sc &c) {
   if (J.pmode == 1) {
-    if (teff(J)) c.Ps -= 0.5 * c.Pv[0] * c.Pv[0] / c.W * J.dthalf;
-    const double V = volume(J, c.L[0], c.L[1], c.L[2]) + c.Pv[0] * c.s / c.W * J.dthalf;
+    if (teff(J)) c.Ps -= 0.5 * c.Pv[0] * c.Pv[0] * c.iW * J.dthalf;
+    const double V = volume(J, c.L[0], c.L[1], c.L[2]) + c.Pv[0] * c.s * c.iW * J.dthalf;
     const double l = cbrt(V);
     c.L[0] = c.L[1] = c.L[2] = l;
   }
   if (J.pmode == 2) {
-    if (teff(J)) c.Ps -= 0.5 * c.Pv[2] * c.Pv[2] / c.W * J.dthalf;
-    c.L[2] += c.s * c.Pv[2] / c.W * J.dthalf;
+    if (teff(J)) c.Ps -= 0.5 * c.Pv[2] * c.Pv[2] * c.iW * J.dthalf;
+    c.L[2] += c.s * c.Pv[2] * c.iW * J.dthalf;
   }
   if (J.pmode == 3) {
-    if (teff(J)) c.Ps -= 0.5 * c.Pv[0] * c.Pv[0] / c.W * J.dthalf;
-    const double A = c.L[0] * c.L[1] + c.s * c.Pv[0] / c.W * J.dthalf;
+    if (teff(J)) c.Ps -= 0.5 * c.Pv[0] * c.Pv[0] * c.iW * J.dthalf;
+    const double A = c.L[0] * c.L[1] + c.s * c.Pv[0] * c.iW * J.dthalf;
     const double l = sqrt(A);
     c.L[0] = c.L[1] = l;
   }
@@ -1041,6 +1046,15 @@ __device__ void first_half(JRef J, double *mol, int N, Sc &c, double *red) {
   const bool T1 = teff(J) == 1;
   const bool needPv = (J.pmode > 0) || T1;
   double sums[4] = {0, 0, 0, 0};  // sum_Pv[3], sum_Ps
+  // PsPv factors (box and s are fixed through the molecule loop)
+  double ipv[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) ipv[d] = 1.0 / (c.L[d] * c.L[d] * c.s * c.s);
+  const double logs = T1 ? log(c.s) : 0.0;
+  double fpv = 0.0;
+  if (J.pmode == 1) fpv = c.s / (volume(J, c.L[0], c.L[1], c.L[2]) * 3.0) * J.dthalf;
+  if (J.pmode == 2) fpv = c.s / c.L[2] * J.dthalf;
+  if (J.pmode == 3) fpv = c.s / (2.0 * c.L[0] * c.L[1]) * J.dthalf;
   {
     const double coef_xi = 0.25 / c.s;
     const double si = 1.0 / c.s;
@@ -1089,18 +1103,17 @@ __device__ void first_half(JRef J, double *mol, int N, Sc &c, double *red) {
     }
   }
 
-  block_sum<NT, 4>(sums, red);  // includes __syncthreads: molecule writes visible
+  // includes __syncthreads (molecule writes visible); the next write of red
+  // follows later workgroup barriers
+  block_sum<NT, 4, false>(sums, red);
   // PsPv (src/integrator.cxx:145-171)
   if (needPv) {
-    for (int d = 0; d < 3; d++) sums[d] /= c.L[d] * c.L[d] * c.s * c.s;
+    for (int d = 0; d < 3; d++) sums[d] *= ipv[d];
   }
-  if (J.pmode == 1) c.Pv[0] += c.s * (sums[0] + sums[1] + sums[2]) / (volume(J, c.L[0], c.L[1], c.L[2]) * 3.0) * J.dthalf;
-  if (J.pmode == 2) c.Pv[2] += c.s * sums[2] / c.L[2] * J.dthalf;
-  if (J.pmode == 3) {
-    const double A = c.L[0] * c.L[1];
-    c.Pv[0] += c.s * (sums[0] + sums[1]) / (2.0 * A) * J.dthalf;
-  }
-  if (T1) c.Ps += (0.25 * (sums[0] + sums[1] + sums[2]) + 2.0 * sums[3] - (1.0 + log(c.s)) * c.gkT + c.H0) * J.dt;
+  if (J.pmode == 1) c.Pv[0] += (sums[0] + sums[1] + sums[2]) * fpv;
+  if (J.pmode == 2) c.Pv[2] += sums[2] * fpv;
+  if (J.pmode == 3) c.Pv[0] += (sums[0] + sums[1]) * fpv;
+  if (T1) c.Ps += (0.25 * (sums[0] + sums[1] + sums[2]) + 2.0 * sums[3] - (1.0 + logs) * c.gkT + c.H0) * J.dt;
   D4(J, c);
 }
 
@@ -1777,7 +1790,9 @@ __device__ bool forces_resident(JRef J, double *mol, int N, Sc &c, const LdsR &S
 #pragma unroll
     for (int u = 0; u < 6; u++) tot[u] += Bk[u * nblk + b];
   }
-  block_sum<NT, 6>(tot, S.red);
+  // the next write of S.red follows the barrier that ends second_half (or
+  // the callers' barriers in the other modes)
+  block_sum<NT, 6, false>(tot, S.red);
   c.lj = 0.5 * tot[0];
   c.clmb = 0.5 * tot[1];
   c.wall = tot[2];
@@ -1828,6 +1843,7 @@ __device__ void run_resident(const LaunchArgs &a, const DescD &R, JRef J, char *
     // InitializeProperty (src/molecules.h:260-270)
     rank_sort_resident<NT, NS>(J, mol, mid, N, Laxis0, 0.0, S, win_dbg);
     if (!forces_resident<NT, NS, PT>(J, mol, N, c, S, g, i0, i1, site_dbg)) return;
+    __syncthreads();  // S.red is read by forces_resident's last reduction
     c.gkT = 6.0 * (double)N * c.T;
     c.H0 = total_energy<NT>(J, mol, N, c, S.red);
     calc_properties<NT>(J, mol, N, c, sc_out, S.red);
@@ -1835,6 +1851,7 @@ __device__ void run_resident(const LaunchArgs &a, const DescD &R, JRef J, char *
     if (a.do_sort) rank_sort_resident<NT, NS>(J, mol, mid, N, Laxis0, a.margin, S, win_dbg);
     else window_from_debug<NT>(gen(R.win), N, S);
     if (!forces_resident<NT, NS, PT>(J, mol, N, c, S, g, i0, i1, site_dbg)) return;
+    __syncthreads();  // S.red is read by forces_resident's last reduction
     calc_properties<NT>(J, mol, N, c, sc_out, S.red);
   } else {
     if (xflag != 0.0) {
