@@ -93,6 +93,9 @@ struct ReplicaDesc {
   double *xbuf;                 // resident path with parts > 1: 2 x nmol x 16 exchange rows
 };
 
+// Resident path: threads per workgroup (4 waves: one per SIMD of a CU).
+constexpr int RNT = 256;
+
 // Resident path: molecules per energy block.  A replica's parts own whole
 // blocks, and the energy/virial totals are sums of block sums.
 constexpr int EB = 4;

@@ -466,14 +466,16 @@ extern "C" int remd_gpu_create(remd_gpu **out, const remd_gpu_desc *d, const rem
   // window fraction along the sort axis ~ 2 (rcut + margin) / L
   const double wf = std::min(1.0, 2.0 * (d->rcut + d->margin) / sc[0].L[g->prm.sort_axis]);
   const int rlds = resident_lds(g->N, g->NS, g->prm.precision, 0);
-  const bool fits = g->nt == 256 && rlds <= lim;
+  const bool fits = (d->block_threads == 0 || d->block_threads == RNT) && rlds <= lim;
   if (d->path == REMD_PATH_RESIDENT && !fits) {
     free_job(g);
     return fail(REMD_EINVAL, "replica does not fit the resident path (" + std::to_string(rlds) + " > " +
-                                 std::to_string(lim) + " bytes of LDS, or block_threads != 256)");
+                                 std::to_string(lim) + " bytes of LDS, or block_threads != " +
+                                 std::to_string(RNT) + ")");
   }
   g->resident = d->path == REMD_PATH_RESIDENT || (d->path == REMD_PATH_AUTO && fits);
   if (g->resident) {
+    g->nt = RNT;
     g->lds = rlds;
     int per_cu = 0, ncu = 0, dev = 0;
     hipGetDevice(&dev);
@@ -492,7 +494,7 @@ extern "C" int remd_gpu_create(remd_gpu **out, const remd_gpu_desc *d, const rem
     const int nb = (g->N + EB - 1) / EB;
     const int M = ((nb + gn - 1) / gn) * EB;  // molecules of the largest part
     const int wcount = std::max(1, (int)std::ceil(wf * g->N));
-    g->prm.chunks = d->chunks ? d->chunks : std::max(1, std::min(std::min(64, 256 / M), wcount));
+    g->prm.chunks = d->chunks ? d->chunks : std::max(1, std::min(std::min(64, RNT / M), wcount));
   } else {
     if (d->chunks && (d->chunks & (d->chunks - 1))) {
       free_job(g);
@@ -1338,7 +1340,7 @@ extern "C" int remd_batch_execute(remd_batch *b) {
   if (!b) return fail(REMD_EINVAL, "null batch");
   if (b->jobs.empty()) return 0;
   for (remd_gpu *g : b->jobs)
-    if (g->nt != b->nt) return fail(REMD_EINVAL, "batched jobs use 256-thread workgroups");
+    if (!g->resident && g->nt != b->nt) return fail(REMD_EINVAL, "batched global-state jobs use 256-thread workgroups");
   if (b->dirty) {
     int rc = rebuild_table(b);
     if (rc) return rc;

@@ -1953,7 +1953,7 @@ __global__ __launch_bounds__(NT) void resident_kernel(LaunchArgs a) {
 // free up.  Workgroups are dispatched in index order, so every split
 // replica has all its parts resident before a queued replica starts, and
 // no part ever waits on a workgroup that is not yet running.
-__global__ __launch_bounds__(256) void resident_batch_kernel(LaunchArgs a) {
+__global__ __launch_bounds__(RNT) void resident_batch_kernel(LaunchArgs a) {
   extern __shared__ __align__(16) char smem[];
   const unsigned e = a.blk[blockIdx.x];
   if (e == ~0u) return;
@@ -1972,12 +1972,12 @@ __global__ __launch_bounds__(256) void resident_batch_kernel(LaunchArgs a) {
   g.xbuf = gen(R.xbuf);
   g.epoch = 0;
   switch ((e >> 8) & 15u) {
-    case 0: run_resident<256, 3, float>(a, R, J, smem, g); break;
-    case 1: run_resident<256, 3, double>(a, R, J, smem, g); break;
-    case 2: run_resident<256, 4, float>(a, R, J, smem, g); break;
-    case 3: run_resident<256, 4, double>(a, R, J, smem, g); break;
-    case 4: run_resident<256, 5, float>(a, R, J, smem, g); break;
-    case 5: run_resident<256, 5, double>(a, R, J, smem, g); break;
+    case 0: run_resident<RNT, 3, float>(a, R, J, smem, g); break;
+    case 1: run_resident<RNT, 3, double>(a, R, J, smem, g); break;
+    case 2: run_resident<RNT, 4, float>(a, R, J, smem, g); break;
+    case 3: run_resident<RNT, 4, double>(a, R, J, smem, g); break;
+    case 4: run_resident<RNT, 5, float>(a, R, J, smem, g); break;
+    case 5: run_resident<RNT, 5, double>(a, R, J, smem, g); break;
     default: break;
   }
 }
@@ -2164,12 +2164,12 @@ static hipError_t launch_nt(const LaunchArgs &a, int nsite, int prec, int lds, h
 // Resident path: block size 256 only.
 static const void *resident_fn(int nsite, int prec) {
   switch (nsite * 2 + prec) {
-    case 6: return (const void *)resident_kernel<256, 3, float>;
-    case 8: return (const void *)resident_kernel<256, 4, float>;
-    case 10: return (const void *)resident_kernel<256, 5, float>;
-    case 7: return (const void *)resident_kernel<256, 3, double>;
-    case 9: return (const void *)resident_kernel<256, 4, double>;
-    case 11: return (const void *)resident_kernel<256, 5, double>;
+    case 6: return (const void *)resident_kernel<RNT, 3, float>;
+    case 8: return (const void *)resident_kernel<RNT, 4, float>;
+    case 10: return (const void *)resident_kernel<RNT, 5, float>;
+    case 7: return (const void *)resident_kernel<RNT, 3, double>;
+    case 9: return (const void *)resident_kernel<RNT, 4, double>;
+    case 11: return (const void *)resident_kernel<RNT, 5, double>;
     default: return nullptr;
   }
 }
@@ -2177,19 +2177,19 @@ static const void *resident_fn(int nsite, int prec) {
 int resident_grid(int nrep, int parts) { return parts <= 1 ? nrep : 8 * parts * ((nrep + 7) / 8); }
 
 int resident_lds(int nmol, int nsite, int prec, int site_dbg) {
-  return resident_lds_bytes(nmol, nsite, prec, 256, site_dbg != 0);
+  return resident_lds_bytes(nmol, nsite, prec, RNT, site_dbg != 0);
 }
 
 hipError_t launch_resident(const LaunchArgs &a, int nsite, int prec, int lds, hipStream_t st) {
   if (a.nrep <= 0) return hipSuccess;
-  const dim3 g(resident_grid(a.nrep, a.parts)), b(256);
+  const dim3 g(resident_grid(a.nrep, a.parts)), b(RNT);
   switch (nsite * 2 + prec) {
-    case 6: hipLaunchKernelGGL((resident_kernel<256, 3, float>), g, b, lds, st, a); break;
-    case 8: hipLaunchKernelGGL((resident_kernel<256, 4, float>), g, b, lds, st, a); break;
-    case 10: hipLaunchKernelGGL((resident_kernel<256, 5, float>), g, b, lds, st, a); break;
-    case 7: hipLaunchKernelGGL((resident_kernel<256, 3, double>), g, b, lds, st, a); break;
-    case 9: hipLaunchKernelGGL((resident_kernel<256, 4, double>), g, b, lds, st, a); break;
-    case 11: hipLaunchKernelGGL((resident_kernel<256, 5, double>), g, b, lds, st, a); break;
+    case 6: hipLaunchKernelGGL((resident_kernel<RNT, 3, float>), g, b, lds, st, a); break;
+    case 8: hipLaunchKernelGGL((resident_kernel<RNT, 4, float>), g, b, lds, st, a); break;
+    case 10: hipLaunchKernelGGL((resident_kernel<RNT, 5, float>), g, b, lds, st, a); break;
+    case 7: hipLaunchKernelGGL((resident_kernel<RNT, 3, double>), g, b, lds, st, a); break;
+    case 9: hipLaunchKernelGGL((resident_kernel<RNT, 4, double>), g, b, lds, st, a); break;
+    case 11: hipLaunchKernelGGL((resident_kernel<RNT, 5, double>), g, b, lds, st, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -2197,12 +2197,12 @@ hipError_t launch_resident(const LaunchArgs &a, int nsite, int prec, int lds, hi
 
 hipError_t launch_resident_batch(const LaunchArgs &a, int lds, hipStream_t st) {
   if (a.nblk <= 0 || !a.blk) return hipSuccess;
-  hipLaunchKernelGGL(resident_batch_kernel, dim3(a.nblk), dim3(256), lds, st, a);
+  hipLaunchKernelGGL(resident_batch_kernel, dim3(a.nblk), dim3(RNT), lds, st, a);
   return hipGetLastError();
 }
 
 hipError_t resident_batch_occupancy(int lds, int *per_cu) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, (const void *)resident_batch_kernel, 256,
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, (const void *)resident_batch_kernel, RNT,
                                                       (size_t)lds);
 }
 
@@ -2210,7 +2210,7 @@ hipError_t resident_batch_occupancy(int lds, int *per_cu) {
 hipError_t resident_occupancy(int nsite, int prec, int lds, int *per_cu) {
   const void *f = resident_fn(nsite, prec);
   if (!f) return hipErrorInvalidValue;
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, f, 256, (size_t)lds);
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, f, RNT, (size_t)lds);
 }
 
 hipError_t launch_interval(const LaunchArgs &a, int nt, int nsite, int prec, int lds, hipStream_t st) {
