@@ -100,6 +100,16 @@ constexpr int RNT = 256;
 // blocks, and the energy/virial totals are sums of block sums.
 constexpr int EB = 4;
 
+// Resident path exchange buffer, per replica and step parity: one 64-byte
+// row per molecule (F[3], n[4]) and one per energy block (lj, clmb, wall,
+// vir[3]).  A part owns whole blocks, so the rows of one cache line have one
+// writer (block rows at part boundaries excepted).
+constexpr int XR = 8;  // doubles per molecule row
+constexpr int XB = 8;  // doubles per energy-block row
+__host__ __device__ inline size_t xbuf_parity_doubles(int nmol) {
+  return (size_t)nmol * XR + (size_t)((nmol + EB - 1) / EB) * XB;
+}
+
 struct LaunchArgs {
   const ReplicaDesc *reps;
   int nrep;

@@ -100,7 +100,6 @@ static int device_lds_limit() {
 // -------------------------------------------------------------- job
 // Largest number of workgroups one replica is split over (resident path).
 constexpr int GMAX = 8;
-constexpr int XW = 16;  // doubles per molecule row of the force exchange
 
 struct remd_gpu {
   remd_gpu_desc desc;
@@ -112,7 +111,7 @@ struct remd_gpu {
   double wfrac = 1.0;        // window fraction of the sort axis, ~2 (rcut + margin) / L
   int last_parts = 1;
   unsigned long long *d_sync = nullptr;  // nrep group words + the abort word
-  double *d_xbuf = nullptr;    // nrep x 2 x N x XW exchange rows
+  double *d_xbuf = nullptr;    // nrep x 2 x xbuf_parity_doubles(N) exchange rows
   unsigned *h_abort = nullptr; // pinned copy of the abort word
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -305,6 +304,9 @@ extern "C" size_t remd_gpu_footprint(const remd_gpu_desc *d) {
   size_t b = R * NFIELD * N * sizeof(double) + R * N * sizeof(int) + R * NSC * sizeof(double);
   b += (size_t)std::max(d->nfwall, 0) * 2 * sizeof(double) + R * sizeof(ReplicaDesc) + sizeof(JobParams);
   b += R * (sizeof(double) * 3 + sizeof(int) * 2 + sizeof(double *)) + 4096;
+  // resident path (an upper bound when the job takes the global-state path):
+  // exchange rows, group words, abort word
+  b += R * 2 * xbuf_parity_doubles((int)N) * sizeof(double) + (R + 1) * sizeof(unsigned long long);
   return b;
 }
 
@@ -360,7 +362,7 @@ static void refresh_reps(remd_gpu *g) {
     R.sc = g->d_sc + (size_t)r * NSC;
     R.site_f = g->debug ? g->d_site + (size_t)r * g->N * g->NS * 3 : nullptr;
     R.win = g->debug ? g->d_win + (size_t)r * g->N * 3 : nullptr;
-    R.xbuf = g->d_xbuf ? g->d_xbuf + (size_t)r * 2 * g->N * XW : nullptr;
+    R.xbuf = g->d_xbuf ? g->d_xbuf + (size_t)r * 2 * xbuf_parity_doubles(g->N) : nullptr;
   }
 }
 
@@ -531,7 +533,7 @@ extern "C" int remd_gpu_create(remd_gpu **out, const remd_gpu_desc *d, const rem
     if (g->resident) {
       HIPCHK(hipMalloc(&g->d_sync, sizeof(unsigned long long) * (R + 1)));
       HIPCHK(hipMemsetAsync(g->d_sync, 0, sizeof(unsigned long long) * (R + 1), g->stream));
-      HIPCHK(hipMalloc(&g->d_xbuf, sizeof(double) * R * 2 * N * XW));
+      HIPCHK(hipMalloc(&g->d_xbuf, sizeof(double) * R * 2 * xbuf_parity_doubles((int)N)));
       HIPCHK(hipHostMalloc(&g->h_abort, sizeof(unsigned), hipHostMallocDefault));
       *g->h_abort = 0u;
     }

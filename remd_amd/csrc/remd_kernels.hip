@@ -1256,8 +1256,6 @@ __global__ __launch_bounds__(NT) void interval_kernel(LaunchArgs a) {
 // energies are reduced from per-molecule values in a fixed order, so results
 // do not depend on G, on batching or on placement.
 
-constexpr int XW = 16;                 // doubles per molecule of the exchange buffer (per parity)
-constexpr int XR = 8;                  // doubles per molecule row: F[3], n[4]
 constexpr unsigned MAXSPIN = 1u << 20; // group-barrier polls before the launch gives up
 
 // Per-item partials of the pair pass: COM force (3), quaternion torque (4),
@@ -1322,7 +1320,7 @@ struct Group {
   unsigned long long *ctr;  // low 32 bits: arrivals of this replica's parts; high 32 bits: 4-bit
                             // count of parts per XCD (added at the first barrier); zero at start
   unsigned *abort;  // nonzero once any wait of the launch timed out
-  double *xbuf;     // 2 x N x XW exchange rows (double-buffered by step parity)
+  double *xbuf;     // 2 x xbuf_parity_doubles(N) exchange rows (double-buffered by step parity)
   unsigned epoch;   // group barriers completed in this launch
   bool same_xcd;    // every part of the replica shares one L2
   int mode;         // same-XCD barrier variant (LaunchArgs::same_xcd_mode)
@@ -1714,7 +1712,7 @@ __device__ void force_range(JRef J, double *mol, int N, const Sc &c, const LdsR 
         for (int u = 0; u < 6; u++) Bk[u * nblk + b] = bs[u];
         if (xblk) {
 #pragma unroll
-          for (int u = 0; u < 6; u++) xblk[(size_t)b * 8 + u] = bs[u];
+          for (int u = 0; u < 6; u++) xblk[(size_t)b * XB + u] = bs[u];
         }
       }
     }
@@ -1739,7 +1737,7 @@ __device__ bool forces_resident(JRef J, double *mol, int N, Sc &c, const LdsR &S
   PT_MARK(4, pt_sc);
   const int nblk = (N + EB - 1) / EB;
   const int b0 = i0 / EB, b1 = (i1 + EB - 1) / EB;
-  double *xrow = g.parts > 1 ? g.xbuf + (size_t)(g.epoch & 1u) * N * XW : nullptr;
+  double *xrow = g.parts > 1 ? g.xbuf + (size_t)(g.epoch & 1u) * xbuf_parity_doubles(N) : nullptr;
   double *xblk = xrow ? xrow + (size_t)N * XR : nullptr;
   double *Bk = S.E;  // 6 x nblk block sums
   force_range<NT, NS, PT>(J, mol, N, c, S, i0, i1, xrow, Bk, nblk, xblk, site_dbg, pt_sc);
@@ -1755,7 +1753,7 @@ __device__ bool forces_resident(JRef J, double *mol, int N, Sc &c, const LdsR &S
       const bool row = k < N && !(k >= i0 && k < i1);
       const bool blk = k < nblk && !(k >= b0 && k < b1);
       const double *o = xrow + (size_t)(row ? k : 0) * XR;
-      const double *ob = xblk + (size_t)(blk ? k : 0) * 8;
+      const double *ob = xblk + (size_t)(blk ? k : 0) * XB;
       double v[7], e[6];
       if (direct) {
 #pragma unroll
@@ -1805,7 +1803,7 @@ __device__ bool forces_resident(JRef J, double *mol, int N, Sc &c, const LdsR &S
 }
 
 template <int NT, int NS, typename PT>
-__device__ void run_resident(const LaunchArgs &a, const DescD &R, JRef J, char *smem,
+__device__ __attribute__((always_inline)) void run_resident(const LaunchArgs &a, const DescD &R, JRef J, char *smem,
                              Group &g) {
   const int N = J.nmol;
   LdsR S = carve_resident<NT, NS>(smem, N, J.precision, R.site_f != nullptr);
@@ -1953,6 +1951,14 @@ __global__ __launch_bounds__(NT) void resident_kernel(LaunchArgs a) {
 // free up.  Workgroups are dispatched in index order, so every split
 // replica has all its parts resident before a queued replica starts, and
 // no part ever waits on a workgroup that is not yet running.
+// The batch kernel calls the six variants as functions (one register
+// allocation each); the single-job kernel inlines its one variant.
+template <int NS, typename PT>
+__device__ __attribute__((noinline)) void run_resident_call(const LaunchArgs &a, const DescD &R, JRef J, char *smem,
+                                                            Group &g) {
+  run_resident<RNT, NS, PT>(a, R, J, smem, g);
+}
+
 __global__ __launch_bounds__(RNT) void resident_batch_kernel(LaunchArgs a) {
   extern __shared__ __align__(16) char smem[];
   const unsigned e = a.blk[blockIdx.x];
@@ -1972,12 +1978,12 @@ __global__ __launch_bounds__(RNT) void resident_batch_kernel(LaunchArgs a) {
   g.xbuf = gen(R.xbuf);
   g.epoch = 0;
   switch ((e >> 8) & 15u) {
-    case 0: run_resident<RNT, 3, float>(a, R, J, smem, g); break;
-    case 1: run_resident<RNT, 3, double>(a, R, J, smem, g); break;
-    case 2: run_resident<RNT, 4, float>(a, R, J, smem, g); break;
-    case 3: run_resident<RNT, 4, double>(a, R, J, smem, g); break;
-    case 4: run_resident<RNT, 5, float>(a, R, J, smem, g); break;
-    case 5: run_resident<RNT, 5, double>(a, R, J, smem, g); break;
+    case 0: run_resident_call<3, float>(a, R, J, smem, g); break;
+    case 1: run_resident_call<3, double>(a, R, J, smem, g); break;
+    case 2: run_resident_call<4, float>(a, R, J, smem, g); break;
+    case 3: run_resident_call<4, double>(a, R, J, smem, g); break;
+    case 4: run_resident_call<5, float>(a, R, J, smem, g); break;
+    case 5: run_resident_call<5, double>(a, R, J, smem, g); break;
     default: break;
   }
 }
