@@ -1270,7 +1270,7 @@ struct LdsR {
   char *region0;    // site records, or the sort scratch
   uint16_t *js, *je;
   double *part;     // NV x NT pair partials, value-major
-  double *E;        // energy-block sums lj, clmb, wall, vir[3] (6 x nblk; sized 6 x N)
+  double *E;        // energy-block sums lj, clmb, wall, vir[3] (6 x nblk)
   double *red;      // block_sum scratch
   int *flag;        // barrier status broadcast (16 ints)
   double *ptimer;   // 16 phase-timer accumulators (REMD_PHASE_TIMERS builds)
@@ -1286,7 +1286,8 @@ __host__ __device__ inline int resident_region0(int nmol, int nsite, int prec) {
 // Bytes of dynamic LDS the resident path needs for one replica.
 __host__ __device__ inline int resident_lds_bytes(int nmol, int nsite, int prec, int nt, bool site_dbg) {
   return align16(NFIELD * 8 * nmol) + align16(4 * nmol) + resident_region0(nmol, nsite, prec) +
-         align16(4 * nmol) + npart(nsite, site_dbg) * 8 * nt + align16(6 * 8 * nmol) + (nt / WAVE) * 32 * 8 + 64 +
+         align16(4 * nmol) + npart(nsite, site_dbg) * 8 * nt + align16(6 * 8 * ((nmol + EB - 1) / EB)) +
+         (nt / WAVE) * 32 * 8 + 64 +
          128;
 }
 
@@ -1306,7 +1307,7 @@ __device__ inline LdsR carve_resident(char *smem, int N, int prec, bool site_dbg
   L.part = (double *)p;
   p += npart(NS, site_dbg) * 8 * NT;
   L.E = (double *)p;
-  p += align16(6 * 8 * N);
+  p += align16(6 * 8 * ((N + EB - 1) / EB));
   L.red = (double *)p;
   p += (NT / WAVE) * 32 * 8;
   L.flag = (int *)p;
