@@ -64,7 +64,7 @@ struct JobParams {
   int chunks;                   // j-chunks per i-molecule in the pair pass (power of two on the
                                 // global-state path, any 1..64 on the resident path)
   int precision;                // 0 mixed (FP32 pair, FP64 acc), 1 FP64
-  int pad0;
+  int compact;                  // 1: screen partners by COM distance before the pair terms
   double dt, dthalf;
   double rcut, rswitch, margin;
   double wall_length;           // A

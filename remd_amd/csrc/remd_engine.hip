@@ -253,6 +253,9 @@ static void fill_params(const remd_gpu_desc *d, JobParams &J) {
   J.sort_interval = d->sort_interval;
   J.nfwall = d->nfwall;
   J.precision = d->precision;
+  // bulk and slit windows are slabs of the box, mostly beyond the cutoff:
+  // screen the partners first; tube windows are mostly inside it
+  J.compact = d->confined == 1 ? 0 : 1;
   J.dt = d->dt;
   J.dthalf = d->dt * 0.5;
   J.rcut = d->rcut;

@@ -717,6 +717,7 @@ __device__ void sort_and_window(JRef J, double *mol, int *mid, int N, double Lax
 template <int NS, typename PT>
 struct PairK {
   PT rc, rl, rc2, rl2, coef, coef30;
+  PT rs2;  // screening radius^2, slightly above rc^2 (pair_acc decides)
   PT sig2, eps4, eps48;
   PT qq[NS][NS];
   PT L[3], Li[3];
@@ -819,6 +820,61 @@ __device__ __forceinline__ void pair_acc(const PT *gi, const PT (&ai)[NS][3], co
   acc.clmb += (double)(sw * ecl);
 }
 
+// The partners of work item (i, c): jj = js + c, js + c + C, ... <= je
+// (indices mod N), each through pair_acc, in this order.  With `compact`
+// the COM distances of the next 32 partners are screened first (radius
+// slightly above the cutoff) and only the ones inside go through pair_acc,
+// still in order: the same pairs and the same sums, with less divergence
+// between the lanes of a wave where most partners lie beyond the cutoff
+// (bulk and slit windows).
+template <int NS, int QB, typename PT>
+__device__ __forceinline__ void pair_window(const PT *rec, int N, int i, int js, int je, int c0, int C,
+                                            bool compact, const PT *gi, const PT (&ai)[NS][3],
+                                            const PairK<NS, PT> &k, Acc<NS> &acc) {
+  constexpr int ST = Rec<NS>::STRIDE;
+  if (!compact) {
+    for (int jj = js + c0; jj <= je; jj += C) {
+      const int j = (jj >= N) ? jj - N : jj;
+      if (j == i) continue;
+      PT rj[3 + 3 * NS];
+      const PT *src = rec + j * ST;
+#pragma unroll
+      for (int u = 0; u < 3 + 3 * NS; u++) rj[u] = src[u];
+      pair_acc<NS, QB, PT>(gi, ai, rj, k, acc);
+    }
+    return;
+  }
+  for (int base = js + c0; base <= je; base += 32 * C) {
+    unsigned mask = 0u;
+    for (int u = 0; u < 32; u++) {
+      const int jj = base + u * C;
+      if (jj > je) break;
+      const int j = (jj >= N) ? jj - N : jj;
+      if (j == i) continue;
+      const PT *src = rec + j * ST;
+      PT dg[3];
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        dg[d] = gi[d] - src[d];
+        if (k.per[d]) dg[d] -= k.L[d] * rnd(dg[d] * k.Li[d]);
+      }
+      const PT r2 = dg[0] * dg[0] + dg[1] * dg[1] + dg[2] * dg[2];
+      if (r2 < k.rs2) mask |= 1u << u;
+    }
+    while (mask) {
+      const int u = __builtin_ctz(mask);
+      mask &= mask - 1u;
+      const int jj = base + u * C;
+      const int j = (jj >= N) ? jj - N : jj;
+      PT rj[3 + 3 * NS];
+      const PT *src = rec + j * ST;
+#pragma unroll
+      for (int v = 0; v < 3 + 3 * NS; v++) rj[v] = src[v];
+      pair_acc<NS, QB, PT>(gi, ai, rj, k, acc);
+    }
+  }
+}
+
 // M2A: LDS records (COM centred on the box, site offsets R(q) b).
 template <int NT, int NS, typename PT>
 __device__ void m2a(JRef J, const double *mol, int N, double L0, double L1, double L2, PT *rec) {
@@ -885,6 +941,7 @@ __device__ void force_pass(JRef J, double *mol, int N, Sc &c, const Lds &S,
 #pragma unroll
       for (int b = 0; b < NS; b++) k.qq[a][b] = (PT)(J.q[a] * J.q[b]);
   }
+  k.rs2 = k.rc2 * (PT)1.0001;
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     k.L[d] = (PT)c.L[d];
@@ -920,17 +977,7 @@ __device__ void force_pass(JRef J, double *mol, int N, Sc &c, const Lds &S,
         ai[a][1] = ri[4 + 3 * a];
         ai[a][2] = ri[5 + 3 * a];
       }
-      const int js = S.js[i];
-      const int je = S.je[i];
-      for (int jj = js + cc; jj <= je; jj += C) {
-        const int j = (jj >= N) ? jj - N : jj;
-        if (j == i) continue;
-        PT rj[3 + 3 * NS];
-        const PT *src = rec + j * ST;
-#pragma unroll
-        for (int u = 0; u < 3 + 3 * NS; u++) rj[u] = src[u];
-        pair_acc<NS, QB, PT>(gi, ai, rj, k, acc);
-      }
+      pair_window<NS, QB, PT>(rec, N, i, S.js[i], S.je[i], cc, C, J.compact != 0, gi, ai, k, acc);
     }
     // fixed xor butterfly over the C lanes of molecule i
     for (int m = 1; m < C; m <<= 1) {
@@ -1531,6 +1578,7 @@ __device__ void force_range(JRef J, double *mol, int N, const Sc &c, const LdsR 
 #pragma unroll
       for (int b = 0; b < NS; b++) k.qq[a][b] = (PT)(J.q[a] * J.q[b]);
   }
+  k.rs2 = k.rc2 * (PT)1.0001;
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     k.L[d] = (PT)c.L[d];
@@ -1590,17 +1638,7 @@ __device__ void force_range(JRef J, double *mol, int N, const Sc &c, const LdsR 
         ai[a][1] = ri[4 + 3 * a];
         ai[a][2] = ri[5 + 3 * a];
       }
-      const int js = S.js[i];
-      const int je = S.je[i];
-      for (int jj = js + cc; jj <= je; jj += C) {
-        const int j = (jj >= N) ? jj - N : jj;
-        if (j == i) continue;
-        PT rj[3 + 3 * NS];
-        const PT *src = rec + j * ST;
-#pragma unroll
-        for (int u = 0; u < 3 + 3 * NS; u++) rj[u] = src[u];
-        pair_acc<NS, QB, PT>(gi, ai, rj, k, acc);
-      }
+      pair_window<NS, QB, PT>(rec, N, i, S.js[i], S.je[i], cc, C, J.compact != 0, gi, ai, k, acc);
       double e_wall = 0.0;
       if (wall) {
         const double f = (f1 - f0) * dri * (wu - dr * widx) + f0;
