@@ -131,6 +131,8 @@ def test_create_rejects_bad_descriptors(field, bad, msg):
     s = inp.config_tube(seed=2, ntemp=1, npress=1)
     d = ra.desc_for(s)
     setattr(d, field, bad)
+    if field == "chunks":
+        d.path = ra.PATH_GLOBAL  # a power of two there; any 1..64 on the resident path
     h = C.c_void_p()
     z = np.zeros(4 * s.nmol)
     rc = ra.lib().remd_gpu_create(C.byref(h), C.byref(d), _scalars(s), ra._dp_of(z), ra._dp_of(z),
