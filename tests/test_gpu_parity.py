@@ -200,11 +200,23 @@ def test_exchange_device_matches_oracle():
 
 
 # ------------------------------------------------------------- paths
+def _path_case(conf):
+    if conf == "tube":
+        return inp.config_tube(seed=2, ntemp=2, npress=2)
+    from remd_amd.workloads import relax
+
+    if conf == "slit":  # partners screened by COM distance on both paths
+        return relax(inp.config_slit(seed=13, nmol=200, ntemp=2, npress=2, Lxy=30.0))
+    return relax(inp.config_bulk(seed=11, ntemp=4))
+
+
+@pytest.mark.parametrize("conf", ["tube", "slit", "bulk"])
 @pytest.mark.parametrize("prec,tol", [(1, 1e-11), (0, 1e-5)])
-def test_resident_path_agrees_with_global_path(prec, tol):
-    """The two paths reduce in different orders; 20 steps of the 64-replica
-    bench tube (config seed 2, 4 replicas) agree to the stated tolerance."""
-    s = inp.config_tube(seed=2, ntemp=2, npress=2)
+def test_resident_path_agrees_with_global_path(prec, tol, conf):
+    """The two paths reduce in different orders; 20 steps of 4 replicas (the
+    bench tube, a 200-TIP5P slit, the 256-TIP4P bulk cube) agree to the
+    stated tolerance."""
+    s = _path_case(conf)
     s.interval = 20
     out = []
     for path in PATHS:
