@@ -139,11 +139,12 @@ struct remd_gpu {
 
 // Every resident replica of a batch runs in ONE launch of
 // resident_batch_kernel (any water model and precision).  Global-state
-// replicas are launched per class = (sites, pair precision), each class on
-// its own stream, after the resident launch, so a long global-state launch
-// never holds the CUs a waiting part needs.
-constexpr int NCLASS = 6;
-static int kclass(int nsite, int prec) { return (nsite - 3) * 2 + prec; }
+// replicas are launched per class = (workgroup size, sites, pair precision),
+// each class on its own stream, after the resident launch, so a long
+// global-state launch never holds the CUs a waiting part needs.
+constexpr int NCLASS = 12;
+static int kclass(int nsite, int prec, int nt) { return (nt == 512 ? 6 : 0) + (nsite - 3) * 2 + prec; }
+static int class_nt(int c) { return c >= 6 ? 512 : 256; }
 
 struct remd_batch {
   std::vector<remd_gpu *> jobs;
@@ -151,7 +152,6 @@ struct remd_batch {
   int table_cap = 0;
   int ntab = 0;
   bool dirty = true;
-  int nt = 256;
   // resident rows
   int res_n = 0, res_lds = 0, res_capacity = 0;
   std::vector<int> row_job;  // resident row -> index into jobs
@@ -466,7 +466,9 @@ extern "C" int remd_gpu_create(remd_gpu **out, const remd_gpu_desc *d, const rem
   g->N = d->nmol;
   g->NS = d->nsite;
   fill_params(d, g->prm);
-  g->nt = d->block_threads ? d->block_threads : 256;
+  // global-state path: 512-thread workgroups (two waves per SIMD) for
+  // replicas of 512 molecules or more, 256 below
+  g->nt = d->block_threads ? d->block_threads : (d->nmol >= 512 ? 512 : 256);
   const int lim = device_lds_limit();
   // window fraction along the sort axis ~ 2 (rcut + margin) / L
   const double wf = std::min(1.0, 2.0 * (d->rcut + d->margin) / sc[0].L[g->prm.sort_axis]);
@@ -798,11 +800,11 @@ extern "C" int remd_gpu_autotune(remd_gpu *g, int reps) {
   if (!g) return fail(REMD_EINVAL, "null handle");
   if (reps < 1) reps = 3;
   if (g->resident) return autotune_resident(g, reps);
-  const int nts[2] = {128, 256};
+  const int nts[3] = {128, 256, 512};
   double best = 1e300;
   int bnt = g->nt, bc = g->prm.chunks;
   const int lim = device_lds_limit();
-  for (int a = 0; a < 2; a++) {
+  for (int a = 0; a < 3; a++) {
     for (int C = 1; C <= 16; C *= 2) {
       const int lds = lds_bytes_host(g->N, g->NS, g->prm.precision, nts[a]);
       if (lds > lim) continue;
@@ -1276,7 +1278,7 @@ static int rebuild_table(remd_batch *b) {
   for (size_t k = 0; k < b->jobs.size(); k++) {
     remd_gpu *g = b->jobs[k];
     refresh_reps(g);
-    const int c = g->resident ? -1 : kclass(g->NS, g->prm.precision);
+    const int c = g->resident ? -1 : kclass(g->NS, g->prm.precision, g->nt);
     const double w = interval_cost(g, 1);
     for (int r = 0; r < g->nrep; r++) rows.push_back({c, -w, (int)k, g->h_reps[r]});
     if (g->resident) {
@@ -1345,7 +1347,8 @@ extern "C" int remd_batch_execute(remd_batch *b) {
   if (!b) return fail(REMD_EINVAL, "null batch");
   if (b->jobs.empty()) return 0;
   for (remd_gpu *g : b->jobs)
-    if (!g->resident && g->nt != b->nt) return fail(REMD_EINVAL, "batched global-state jobs use 256-thread workgroups");
+    if (!g->resident && g->nt != 256 && g->nt != 512)
+      return fail(REMD_EINVAL, "batched global-state jobs use 256- or 512-thread workgroups");
   if (b->dirty) {
     int rc = rebuild_table(b);
     if (rc) return rc;
@@ -1392,8 +1395,8 @@ extern "C" int remd_batch_execute(remd_batch *b) {
     a.same_xcd_mode = 0;
     a.sync = nullptr;
     a.abort = nullptr;
-    const int nsite = 3 + c / 2, prec = c % 2;
-    HIPCHK(launch_interval(a, b->nt, nsite, prec, b->cls_lds[c], b->cstream[c]));
+    const int nsite = 3 + (c % 6) / 2, prec = c % 2;
+    HIPCHK(launch_interval(a, class_nt(c), nsite, prec, b->cls_lds[c], b->cstream[c]));
   }
   // join
   for (int c = 0; c < NCLASS; c++) {
