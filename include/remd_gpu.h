@@ -61,14 +61,17 @@ typedef struct {
   double site_m[5], site_sigma[5], site_eps[5], site_q[5];
   double mass, inertia[3];
   int precision;         /* REMD_PREC_MIXED: FP32 pair math, FP64 accumulation */
-  int block_threads;     /* 0: default (256); 128, 256 or 512 */
+  int block_threads;     /* 0: default; 128, 256 or 512 (global-state path) */
   int chunks;            /* j-chunks per molecule in the pair pass; 0: heuristic */
   int path;              /* REMD_PATH_AUTO, _RESIDENT (state in LDS, replicas may span
-                            several workgroups) or _GLOBAL (state in HBM, one workgroup
-                            per replica); AUTO takes RESIDENT when the replica fits */
+                            several workgroups), _SPLIT (as RESIDENT with the state in a
+                            private global copy per workgroup, for replicas too large for
+                            the LDS) or _GLOBAL (state in HBM, one workgroup per replica);
+                            AUTO takes RESIDENT when the replica fits, else SPLIT when it
+                            gets 4 or more workgroups alone, else GLOBAL */
 } remd_gpu_desc;
 
-enum { REMD_PATH_AUTO = 0, REMD_PATH_RESIDENT = 1, REMD_PATH_GLOBAL = 2 };
+enum { REMD_PATH_AUTO = 0, REMD_PATH_RESIDENT = 1, REMD_PATH_GLOBAL = 2, REMD_PATH_SPLIT = 3 };
 
 /* Per-replica extended variables (src/variables.h:137-157) and box. */
 typedef struct {

@@ -129,7 +129,7 @@ def _ip_of(a):
     return a.ctypes.data_as(_ip)
 
 
-PATH_AUTO, PATH_RESIDENT, PATH_GLOBAL = 0, 1, 2
+PATH_AUTO, PATH_RESIDENT, PATH_GLOBAL, PATH_SPLIT = 0, 1, 2, 3
 
 
 def desc_for(system: "inputs.System", precision=0, block_threads=0, chunks=0, path=PATH_AUTO) -> Desc:

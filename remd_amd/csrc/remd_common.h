@@ -90,7 +90,8 @@ struct ReplicaDesc {
   double *sc;                   // NSC doubles
   double *site_f;               // optional: nsite*nmol*3 (last force pass)
   int *win;                     // optional: 3*nmol window (js, je, whole-ring flag)
-  double *xbuf;                 // resident path with parts > 1: 2 x nmol x 16 exchange rows
+  double *xbuf;                 // resident path with parts > 1: exchange rows (xbuf_parity_doubles)
+  double *gbuf;                 // split-global path: one private state copy per part (gm_part_doubles)
 };
 
 // Resident path: threads per workgroup (4 waves: one per SIMD of a CU).
@@ -108,6 +109,12 @@ constexpr int XR = 8;  // doubles per molecule row
 constexpr int XB = 8;  // doubles per energy-block row
 __host__ __device__ inline size_t xbuf_parity_doubles(int nmol) {
   return (size_t)nmol * XR + (size_t)((nmol + EB - 1) / EB) * XB;
+}
+
+// Split-global path: a part's private copy of the replica state in global
+// memory (NFIELD x nmol doubles, then nmol ints), in doubles.
+__host__ __device__ inline size_t gm_part_doubles(int nmol) {
+  return (size_t)NFIELD * nmol + (size_t)((nmol + 1) / 2);
 }
 
 struct LaunchArgs {

@@ -24,7 +24,7 @@ hipError_t launch_resident(const LaunchArgs &a, int nsite, int prec, int lds, hi
 hipError_t resident_occupancy(int nsite, int prec, int lds, int *per_cu);
 hipError_t launch_resident_batch(const LaunchArgs &a, int lds, hipStream_t st);
 hipError_t resident_batch_occupancy(int lds, int *per_cu);
-int resident_lds(int nmol, int nsite, int prec, int site_dbg);
+int resident_lds(int nmol, int nsite, int prec, int site_dbg, int gm);
 hipError_t launch_wall_tube(int nfwall, double R, double sgm06, double coefE, double *fw, hipStream_t st);
 hipError_t launch_wall_slit(int nfwall, double H, double sgm, double eps, double rho, double *fw,
                             hipStream_t st);
@@ -107,6 +107,10 @@ struct remd_gpu {
   int nrep = 0, N = 0, NS = 0;
   int nt = 256, lds = 0;
   bool resident = false;     // state in LDS for the launch (resident_kernel)
+  bool gm = false;           // resident kernel, split-global variant: state in per-part global copies
+  double *d_gbuf = nullptr;  // split-global: nrep x GMAX x gm_part_doubles(N)
+  unsigned *d_blk1 = nullptr;  // split-global: workgroup table of the last (nrep, parts) launch shape
+  int blk1_nrep = 0, blk1_parts = 0, blk1_n = 0;
   int capacity = 0;          // resident workgroups the device holds at once
   double wfrac = 1.0;        // window fraction of the sort axis, ~2 (rcut + margin) / L
   int last_parts = 1;
@@ -222,7 +226,8 @@ static int validate(const remd_gpu_desc *d, const remd_gpu_scalars *sc) {
   if (d->block_threads != 0 && d->block_threads != 128 && d->block_threads != 256 && d->block_threads != 512)
     return fail(REMD_EINVAL, "block_threads must be 0, 128, 256 or 512");
   if (d->chunks < 0 || d->chunks > 64) return fail(REMD_EINVAL, "chunks must be in [0, 64]");
-  if (d->path < 0 || d->path > 2) return fail(REMD_EINVAL, "path must be 0 (auto), 1 (resident) or 2 (global)");
+  if (d->path < 0 || d->path > 3)
+    return fail(REMD_EINVAL, "path must be 0 (auto), 1 (resident), 2 (global) or 3 (split)");
   if (d->precision != 0 && d->precision != 1) return fail(REMD_EINVAL, "precision must be 0 or 1");
   const int axis = (Cm == 2) ? 0 : 2;
   for (int r = 0; r < d->nreplica; r++) {
@@ -310,6 +315,8 @@ extern "C" size_t remd_gpu_footprint(const remd_gpu_desc *d) {
   // resident path (an upper bound when the job takes the global-state path):
   // exchange rows, group words, abort word
   b += R * 2 * xbuf_parity_doubles((int)N) * sizeof(double) + (R + 1) * sizeof(unsigned long long);
+  // split-global private copies (replicas of 512 or more molecules may take that path)
+  if (N >= 512) b += R * GMAX * gm_part_doubles((int)N) * sizeof(double);
   return b;
 }
 
@@ -332,6 +339,8 @@ static void free_job(remd_gpu *g) {
   hipFree(g->d_xjob);
   hipFree(g->d_sync);
   hipFree(g->d_xbuf);
+  hipFree(g->d_gbuf);
+  hipFree(g->d_blk1);
   if (g->h_abort) hipHostFree(g->h_abort);
   if (g->h_sc) hipHostFree(g->h_sc);
   if (g->ev0) hipEventDestroy(g->ev0);
@@ -366,6 +375,7 @@ static void refresh_reps(remd_gpu *g) {
     R.site_f = g->debug ? g->d_site + (size_t)r * g->N * g->NS * 3 : nullptr;
     R.win = g->debug ? g->d_win + (size_t)r * g->N * 3 : nullptr;
     R.xbuf = g->d_xbuf ? g->d_xbuf + (size_t)r * 2 * xbuf_parity_doubles(g->N) : nullptr;
+    R.gbuf = g->d_gbuf ? g->d_gbuf + (size_t)r * GMAX * gm_part_doubles(g->N) : nullptr;
   }
 }
 
@@ -408,6 +418,47 @@ static int pick_parts(int capacity, int nrep, int N) {
   return p >= 2 ? p : 1;
 }
 
+// A resident-path launch of nrep replicas with a.parts workgroups each: the
+// LDS kernel, or for the split-global variant the table-driven kernel with
+// the same workgroup mapping (parts of a replica on block indices equal
+// mod 8).
+static int launch_res(remd_gpu *g, LaunchArgs &a, hipStream_t st) {
+  if (!g->gm) {
+    HIPCHK(launch_resident(a, g->NS, g->prm.precision, g->lds, st));
+    return 0;
+  }
+  const int parts = a.parts < 1 ? 1 : a.parts;
+  if (g->blk1_nrep != a.nrep || g->blk1_parts != parts || !g->d_blk1) {
+    const int nb = parts <= 1 ? a.nrep : 8 * parts * ((a.nrep + 7) / 8);
+    std::vector<unsigned> blk(nb, ~0u);
+    const unsigned v = (unsigned)(6 + (g->NS - 3) * 2 + g->prm.precision);
+    for (int b = 0; b < nb; b++) {
+      int rep, part;
+      if (parts <= 1) {
+        rep = b;
+        part = 0;
+      } else {
+        const int slot = b >> 3;
+        rep = (slot / parts) * 8 + (b & 7);
+        part = slot % parts;
+      }
+      if (rep < a.nrep) blk[b] = (unsigned)rep << 12 | v << 8 | (unsigned)parts << 4 | (unsigned)part;
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    hipFree(g->d_blk1);
+    g->d_blk1 = nullptr;
+    HIPCHK(hipMalloc(&g->d_blk1, sizeof(unsigned) * nb));
+    HIPCHK(hipMemcpy(g->d_blk1, blk.data(), sizeof(unsigned) * nb, hipMemcpyHostToDevice));
+    g->blk1_nrep = a.nrep;
+    g->blk1_parts = parts;
+    g->blk1_n = nb;
+  }
+  a.blk = g->d_blk1;
+  a.nblk = g->blk1_n;
+  HIPCHK(launch_resident_batch(a, g->lds, st));
+  return 0;
+}
+
 static int launch(remd_gpu *g, int mode, int do_sort, double margin, const ReplicaDesc *reps, int nrep,
                   bool timed) {
   LaunchArgs a;
@@ -427,8 +478,12 @@ static int launch(remd_gpu *g, int mode, int do_sort, double margin, const Repli
   }
   g->last_parts = a.parts;
   if (timed) HIPCHK(hipEventRecord(g->ev0, g->stream));
-  if (g->resident) HIPCHK(launch_resident(a, g->NS, g->prm.precision, g->lds, g->stream));
-  else HIPCHK(launch_interval(a, g->nt, g->NS, g->prm.precision, g->lds, g->stream));
+  if (g->resident) {
+    int rc = launch_res(g, a, g->stream);
+    if (rc) return rc;
+  } else {
+    HIPCHK(launch_interval(a, g->nt, g->NS, g->prm.precision, g->lds, g->stream));
+  }
   if (timed) HIPCHK(hipEventRecord(g->ev1, g->stream));
   if (a.parts > 1)
     HIPCHK(hipMemcpyAsync(g->h_abort, a.abort, sizeof(unsigned), hipMemcpyDeviceToHost, g->stream));
@@ -472,21 +527,45 @@ extern "C" int remd_gpu_create(remd_gpu **out, const remd_gpu_desc *d, const rem
   const int lim = device_lds_limit();
   // window fraction along the sort axis ~ 2 (rcut + margin) / L
   const double wf = std::min(1.0, 2.0 * (d->rcut + d->margin) / sc[0].L[g->prm.sort_axis]);
-  const int rlds = resident_lds(g->N, g->NS, g->prm.precision, 0);
-  const bool fits = (d->block_threads == 0 || d->block_threads == RNT) && rlds <= lim;
+  const bool bt_ok = d->block_threads == 0 || d->block_threads == RNT;
+  const int rlds = resident_lds(g->N, g->NS, g->prm.precision, 0, 0);
+  const int glds = resident_lds(g->N, g->NS, g->prm.precision, 0, 1);
+  const bool fits = bt_ok && rlds <= lim;
+  const bool fits_gm = bt_ok && glds <= lim;
   if (d->path == REMD_PATH_RESIDENT && !fits) {
     free_job(g);
     return fail(REMD_EINVAL, "replica does not fit the resident path (" + std::to_string(rlds) + " > " +
                                  std::to_string(lim) + " bytes of LDS, or block_threads != " +
                                  std::to_string(RNT) + ")");
   }
-  g->resident = d->path == REMD_PATH_RESIDENT || (d->path == REMD_PATH_AUTO && fits);
+  if (d->path == REMD_PATH_SPLIT && !fits_gm) {
+    free_job(g);
+    return fail(REMD_EINVAL, "replica does not fit the split-global path (" + std::to_string(glds) + " > " +
+                                 std::to_string(lim) + " bytes of LDS, or block_threads != " +
+                                 std::to_string(RNT) + ")");
+  }
+  int per_cu_gm = 0, ncu_dev = 0;
+  if (d->path == REMD_PATH_AUTO && !fits && fits_gm) {
+    // AUTO takes the split-global variant for a replica too large for the
+    // LDS when it would get 4 or more workgroups alone; with fewer the
+    // global-state path's 512-thread workgroups are faster
+    int dev = 0;
+    hipGetDevice(&dev);
+    if (resident_batch_occupancy(glds, &per_cu_gm) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu_dev, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      per_cu_gm = ncu_dev = 0;
+  }
+  const bool auto_gm = per_cu_gm > 0 && ncu_dev > 0 && pick_parts(per_cu_gm * ncu_dev, g->nrep, g->N) >= 4;
+  g->gm = d->path == REMD_PATH_SPLIT || (d->path == REMD_PATH_AUTO && !fits && auto_gm);
+  g->resident = d->path == REMD_PATH_RESIDENT || (d->path == REMD_PATH_AUTO && fits) || g->gm;
   if (g->resident) {
     g->nt = RNT;
-    g->lds = rlds;
+    g->lds = g->gm ? glds : rlds;
     int per_cu = 0, ncu = 0, dev = 0;
     hipGetDevice(&dev);
-    if (resident_occupancy(g->NS, g->prm.precision, rlds, &per_cu) != hipSuccess ||
+    const hipError_t oe = g->gm ? resident_batch_occupancy(g->lds, &per_cu)
+                                : resident_occupancy(g->NS, g->prm.precision, g->lds, &per_cu);
+    if (oe != hipSuccess ||
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || per_cu < 1 ||
         ncu < 1) {
       free_job(g);
@@ -539,6 +618,7 @@ extern "C" int remd_gpu_create(remd_gpu **out, const remd_gpu_desc *d, const rem
       HIPCHK(hipMalloc(&g->d_sync, sizeof(unsigned long long) * (R + 1)));
       HIPCHK(hipMemsetAsync(g->d_sync, 0, sizeof(unsigned long long) * (R + 1), g->stream));
       HIPCHK(hipMalloc(&g->d_xbuf, sizeof(double) * R * 2 * xbuf_parity_doubles((int)N)));
+      if (g->gm) HIPCHK(hipMalloc(&g->d_gbuf, sizeof(double) * R * GMAX * gm_part_doubles((int)N)));
       HIPCHK(hipHostMalloc(&g->h_abort, sizeof(unsigned), hipHostMallocDefault));
       *g->h_abort = 0u;
     }
@@ -659,8 +739,12 @@ extern "C" int remd_gpu_execute_steps_plain(remd_gpu *g, const double *T, const 
     a.same_xcd_mode = same_xcd_barrier_mode();
     a.sync = g->d_sync ? g->d_sync + r : nullptr;
     a.abort = g->d_sync ? (unsigned *)(g->d_sync + g->nrep) : nullptr;
-    if (g->resident) HIPCHK(launch_resident(a, g->NS, g->prm.precision, g->lds, g->stream));
-    else HIPCHK(launch_interval(a, g->nt, g->NS, g->prm.precision, g->lds, g->stream));
+    if (g->resident) {
+      rc = launch_res(g, a, g->stream);
+      if (rc) return rc;
+    } else {
+      HIPCHK(launch_interval(a, g->nt, g->NS, g->prm.precision, g->lds, g->stream));
+    }
   }
   HIPCHK(hipEventRecord(g->ev1, g->stream));
   if (parts > 1)
@@ -696,7 +780,7 @@ extern "C" int remd_gpu_set_debug(remd_gpu *g, int on) {
   }
   if (g->resident) {
     // the site-force partials take extra LDS on the resident path
-    const int lds = resident_lds(g->N, g->NS, g->prm.precision, on != 0);
+    const int lds = resident_lds(g->N, g->NS, g->prm.precision, on != 0, g->gm ? 1 : 0);
     if (lds > device_lds_limit()) return fail(REMD_EINVAL, "site-force debug buffers do not fit this replica's LDS");
     g->lds = lds;
   }
@@ -750,7 +834,7 @@ extern "C" int remd_gpu_launch_shape(remd_gpu *g, int *bt, int *ch, int *lds) {
 
 extern "C" int remd_gpu_launch_path(remd_gpu *g, int *path, int *parts) {
   if (!g) return fail(REMD_EINVAL, "null handle");
-  if (path) *path = g->resident ? REMD_PATH_RESIDENT : REMD_PATH_GLOBAL;
+  if (path) *path = g->gm ? REMD_PATH_SPLIT : (g->resident ? REMD_PATH_RESIDENT : REMD_PATH_GLOBAL);
   if (parts) *parts = g->last_parts;
   return 0;
 }
@@ -1148,7 +1232,9 @@ static double step_cost(const remd_gpu *g, int p) {
 
 static double interval_cost(const remd_gpu *g, int p) { return step_cost(g, p) * std::max(1, g->desc.interval); }
 
-static unsigned variant_of(const remd_gpu *g) { return (unsigned)((g->NS - 3) * 2 + g->prm.precision); }
+static unsigned variant_of(const remd_gpu *g) {
+  return (unsigned)((g->gm ? 6 : 0) + (g->NS - 3) * 2 + g->prm.precision);
+}
 
 // Estimated time of one batched interval under plan P: the split replicas
 // hold their workgroups for their (longest) interval; the single-part

@@ -42,6 +42,7 @@ struct DescD {
   REMD_AS_GLOBAL double *site_f;
   REMD_AS_GLOBAL int *win;
   REMD_AS_GLOBAL double *xbuf;
+  REMD_AS_GLOBAL double *gbuf;
 };
 static_assert(sizeof(DescD) == sizeof(ReplicaDesc), "device view of ReplicaDesc");
 template <class T>
@@ -1330,22 +1331,30 @@ __host__ __device__ inline int resident_region0(int nmol, int nsite, int prec) {
   return align16(rec > sortb ? rec : sortb);
 }
 
-// Bytes of dynamic LDS the resident path needs for one replica.
-__host__ __device__ inline int resident_lds_bytes(int nmol, int nsite, int prec, int nt, bool site_dbg) {
-  return align16(NFIELD * 8 * nmol) + align16(4 * nmol) + resident_region0(nmol, nsite, prec) +
+// Bytes of dynamic LDS the resident path needs for one replica (gm: the
+// split-global variant, whose state lives in global memory).
+__host__ __device__ inline int resident_lds_bytes(int nmol, int nsite, int prec, int nt, bool site_dbg,
+                                                  bool gm = false) {
+  return (gm ? 0 : align16(NFIELD * 8 * nmol) + align16(4 * nmol)) + resident_region0(nmol, nsite, prec) +
          align16(4 * nmol) + npart(nsite, site_dbg) * 8 * nt + align16(6 * 8 * ((nmol + EB - 1) / EB)) +
          (nt / WAVE) * 32 * 8 + 64 +
          128;
 }
 
 template <int NT, int NS>
-__device__ inline LdsR carve_resident(char *smem, int N, int prec, bool site_dbg) {
+__device__ inline LdsR carve_resident(char *smem, int N, int prec, bool site_dbg, double *gmol = nullptr,
+                                      int *gmid = nullptr) {
   LdsR L;
   char *p = smem;
-  L.mol = (double *)p;
-  p += align16(NFIELD * 8 * N);
-  L.mid = (int *)p;
-  p += align16(4 * N);
+  if (gmol) {
+    L.mol = gmol;
+    L.mid = gmid;
+  } else {
+    L.mol = (double *)p;
+    p += align16(NFIELD * 8 * N);
+    L.mid = (int *)p;
+    p += align16(4 * N);
+  }
   L.region0 = p;
   p += resident_region0(N, NS, prec);
   L.js = (uint16_t *)p;
@@ -1841,11 +1850,19 @@ __device__ bool forces_resident(JRef J, double *mol, int N, Sc &c, const LdsR &S
   return true;
 }
 
-template <int NT, int NS, typename PT>
+// GM (split-global variant): the replica's state is a private copy per
+// part in global memory (R.gbuf) instead of LDS; everything else as above.
+template <int NT, int NS, typename PT, bool GM = false>
 __device__ __attribute__((always_inline)) void run_resident(const LaunchArgs &a, const DescD &R, JRef J, char *smem,
                              Group &g) {
   const int N = J.nmol;
-  LdsR S = carve_resident<NT, NS>(smem, N, J.precision, R.site_f != nullptr);
+  double *gmol = nullptr;
+  int *gmid = nullptr;
+  if (GM) {
+    gmol = gen(R.gbuf) + (size_t)g.part * gm_part_doubles(N);
+    gmid = (int *)(gmol + (size_t)NFIELD * N);
+  }
+  LdsR S = carve_resident<NT, NS>(smem, N, J.precision, R.site_f != nullptr, gmol, gmid);
   double *mol = S.mol;
   int *mid = S.mid;
   for (int k = threadIdx.x; k < NFIELD * N; k += NT) mol[k] = gen(R.mol)[k];
@@ -1982,7 +1999,8 @@ __global__ __launch_bounds__(NT) void resident_kernel(LaunchArgs a) {
 
 // One launch for every resident replica of a batch, whatever its water
 // model and pair precision.  Workgroup b runs the entry blk[b] = rep << 12 |
-// variant << 8 | parts << 4 | part (variant = 2 (nsite - 3) + prec), or
+// variant << 8 | parts << 4 | part (variant = 2 (nsite - 3) + prec, plus 6
+// for the split-global variant whose state is in global memory), or
 // nothing for ~0u.  The host lists the parts of every split replica first,
 // the parts of one replica on block indices equal mod 8 (one XCD), all of
 // them within the workgroups the device holds at once; the single-part
@@ -1990,12 +2008,12 @@ __global__ __launch_bounds__(NT) void resident_kernel(LaunchArgs a) {
 // free up.  Workgroups are dispatched in index order, so every split
 // replica has all its parts resident before a queued replica starts, and
 // no part ever waits on a workgroup that is not yet running.
-// The batch kernel calls the six variants as functions (one register
+// The batch kernel calls the twelve variants as functions (one register
 // allocation each); the single-job kernel inlines its one variant.
-template <int NS, typename PT>
+template <int NS, typename PT, bool GM>
 __device__ __attribute__((noinline)) void run_resident_call(const LaunchArgs &a, const DescD &R, JRef J, char *smem,
                                                             Group &g) {
-  run_resident<RNT, NS, PT>(a, R, J, smem, g);
+  run_resident<RNT, NS, PT, GM>(a, R, J, smem, g);
 }
 
 __global__ __launch_bounds__(RNT) void resident_batch_kernel(LaunchArgs a) {
@@ -2017,12 +2035,18 @@ __global__ __launch_bounds__(RNT) void resident_batch_kernel(LaunchArgs a) {
   g.xbuf = gen(R.xbuf);
   g.epoch = 0;
   switch ((e >> 8) & 15u) {
-    case 0: run_resident_call<3, float>(a, R, J, smem, g); break;
-    case 1: run_resident_call<3, double>(a, R, J, smem, g); break;
-    case 2: run_resident_call<4, float>(a, R, J, smem, g); break;
-    case 3: run_resident_call<4, double>(a, R, J, smem, g); break;
-    case 4: run_resident_call<5, float>(a, R, J, smem, g); break;
-    case 5: run_resident_call<5, double>(a, R, J, smem, g); break;
+    case 0: run_resident_call<3, float, false>(a, R, J, smem, g); break;
+    case 6: run_resident_call<3, float, true>(a, R, J, smem, g); break;
+    case 1: run_resident_call<3, double, false>(a, R, J, smem, g); break;
+    case 7: run_resident_call<3, double, true>(a, R, J, smem, g); break;
+    case 2: run_resident_call<4, float, false>(a, R, J, smem, g); break;
+    case 8: run_resident_call<4, float, true>(a, R, J, smem, g); break;
+    case 3: run_resident_call<4, double, false>(a, R, J, smem, g); break;
+    case 9: run_resident_call<4, double, true>(a, R, J, smem, g); break;
+    case 4: run_resident_call<5, float, false>(a, R, J, smem, g); break;
+    case 10: run_resident_call<5, float, true>(a, R, J, smem, g); break;
+    case 5: run_resident_call<5, double, false>(a, R, J, smem, g); break;
+    case 11: run_resident_call<5, double, true>(a, R, J, smem, g); break;
     default: break;
   }
 }
@@ -2221,8 +2245,8 @@ static const void *resident_fn(int nsite, int prec) {
 
 int resident_grid(int nrep, int parts) { return parts <= 1 ? nrep : 8 * parts * ((nrep + 7) / 8); }
 
-int resident_lds(int nmol, int nsite, int prec, int site_dbg) {
-  return resident_lds_bytes(nmol, nsite, prec, RNT, site_dbg != 0);
+int resident_lds(int nmol, int nsite, int prec, int site_dbg, int gm) {
+  return resident_lds_bytes(nmol, nsite, prec, RNT, site_dbg != 0, gm != 0);
 }
 
 hipError_t launch_resident(const LaunchArgs &a, int nsite, int prec, int lds, hipStream_t st) {

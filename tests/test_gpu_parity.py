@@ -64,7 +64,7 @@ def test_sort_window_bitexact(n, w, seed):
 
 
 # ------------------------------------------------------------- a1..a5, b2
-PATHS = [ra.PATH_RESIDENT, ra.PATH_GLOBAL]
+PATHS = [ra.PATH_RESIDENT, ra.PATH_GLOBAL, ra.PATH_SPLIT]
 
 
 def bulk_large(n=600, L=26.2, seed=41, model="TIP4P", ntemp=2):
@@ -213,9 +213,9 @@ def _path_case(conf):
 @pytest.mark.parametrize("conf", ["tube", "slit", "bulk"])
 @pytest.mark.parametrize("prec,tol", [(1, 1e-11), (0, 1e-5)])
 def test_resident_path_agrees_with_global_path(prec, tol, conf):
-    """The two paths reduce in different orders; 20 steps of 4 replicas (the
-    bench tube, a 200-TIP5P slit, the 256-TIP4P bulk cube) agree to the
-    stated tolerance."""
+    """The paths (resident, global-state, split-global) reduce in different
+    orders; 20 steps of 4 replicas (the bench tube, a 200-TIP5P slit, the
+    256-TIP4P bulk cube) agree to the stated tolerance."""
     s = _path_case(conf)
     s.interval = 20
     out = []
@@ -224,25 +224,61 @@ def test_resident_path_agrees_with_global_path(prec, tol, conf):
         g.execute_steps(s.T, s.P)
         out.append((g.all_averages(), [g.state(k) for k in range(4)]))
         g.close()
-    (a0, s0), (a1, s1) = out
-    for k in range(4):
-        assert np.array_equal(s0[k]["id"], s1[k]["id"])
-        dd = s0[k]["r"] - s1[k]["r"]
+    a0, s0 = out[0]
+    for a1, s1 in out[1:]:
+        for k in range(4):
+            assert np.array_equal(s0[k]["id"], s1[k]["id"])
+            dd = s0[k]["r"] - s1[k]["r"]
+            dd -= np.rint(dd)
+            assert np.max(np.abs(dd)) <= tol
+            assert rel_rms(s0[k]["p"], s1[k]["p"]) <= 1e3 * tol
+        assert np.allclose(a0[:, :3], a1[:, :3], rtol=1e3 * tol, atol=1e-6)
+
+
+def test_large_replica_takes_split_path():
+    """700 TIP4P (a slit, 16 replicas) do not fit the LDS: AUTO takes the
+    split-global path; its step-0 forces match the oracle and 20 steps agree
+    with the global-state path."""
+    from remd_amd.workloads import relax
+
+    s = relax(inp.config_slit(seed=61, nmol=700, Lxy=55.0, ntemp=4, npress=4, model="TIP4P"))
+    s.interval = 20
+    g = ra.REMDGPU(s)
+    path, parts = g.launch_path()
+    print("path", path, "parts", parts)
+    assert path == ra.PATH_SPLIT
+    sg = g.state(0)
+    o = oracle_for(s, 0)
+    o.initialize_property()
+    so = o.state()
+    assert np.array_equal(sg["id"], so["id"])
+    assert rel_rms(sg["f"], so["f"]) <= 1e-5 and rel_rms(sg["n"], so["n"]) <= 1e-5
+    g.execute_steps(s.T, s.P)
+    assert g.launch_path()[1] >= 4
+    h = ra.REMDGPU(s, path=ra.PATH_GLOBAL)
+    h.execute_steps(s.T, s.P)
+    for k in (0, s.nreplica - 1):
+        a, b = g.state(k), h.state(k)
+        assert np.array_equal(a["id"], b["id"])
+        dd = a["r"] - b["r"]
         dd -= np.rint(dd)
-        assert np.max(np.abs(dd)) <= tol
-        assert rel_rms(s0[k]["p"], s1[k]["p"]) <= 1e3 * tol
-    assert np.allclose(a0[:, :3], a1[:, :3], rtol=1e3 * tol, atol=1e-6)
+        assert np.max(np.abs(dd)) <= 1e-5
+    assert np.allclose(g.all_averages()[:, :3], h.all_averages()[:, :3], rtol=1e-2, atol=1e-6)
+    g.close()
+    h.close()
 
 
-def test_parts_do_not_change_results():
+@pytest.mark.parametrize("path", [ra.PATH_RESIDENT, ra.PATH_SPLIT])
+def test_parts_do_not_change_results(path):
     """A replica split over 7 workgroups (alone) gives the same bits as in a
     batch with a 254-replica filler job, over three intervals run with the
     batch's plan fixed to 1 and to 3 workgroups per replica, then chosen by
-    the autotuner."""
+    the autotuner (state in LDS, or in per-workgroup global copies)."""
     a = inp.config_tube(seed=31, nmol=60, Lz=26.0, ntemp=2, npress=1, nfwall=500)
     a.interval = 20
     a.dt_fs = 0.5  # unrelaxed random placement: short steps keep it finite
-    g = ra.REMDGPU(a)
+    g = ra.REMDGPU(a, path=path)
+    assert g.launch_path()[0] == path
     for _ in range(3):
         g.execute_steps(a.T, a.P)
     parts_alone = g.launch_path()[1]
@@ -251,7 +287,7 @@ def test_parts_do_not_change_results():
     g.close()
     filler = inp.config_tube(seed=33, nmol=16, Lz=22.0, ntemp=127, npress=2, nfwall=300)
     filler.interval = 20
-    ga, gf = ra.REMDGPU(a), ra.REMDGPU(filler)
+    ga, gf = ra.REMDGPU(a, path=path), ra.REMDGPU(filler)
     bt = ra.Batch()
     bt.add(ga)
     bt.add(gf)
