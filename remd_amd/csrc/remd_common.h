@@ -112,9 +112,10 @@ __host__ __device__ inline size_t xbuf_parity_doubles(int nmol) {
 }
 
 // Split-global path: a part's private copy of the replica state in global
-// memory (NFIELD x nmol doubles, then nmol ints), in doubles.
+// memory (NFIELD x nmol doubles, the energy-block sums 6 x nblk, then nmol
+// ints), in doubles.
 __host__ __device__ inline size_t gm_part_doubles(int nmol) {
-  return (size_t)NFIELD * nmol + (size_t)((nmol + 1) / 2);
+  return (size_t)NFIELD * nmol + (size_t)6 * ((nmol + EB - 1) / EB) + (size_t)((nmol + 1) / 2);
 }
 
 struct LaunchArgs {

@@ -1318,6 +1318,7 @@ struct LdsR {
   char *region0;    // site records, or the sort scratch
   uint16_t *js, *je;
   double *part;     // NV x NT pair partials, value-major
+  int part_n;       // doubles in the partial area
   double *E;        // energy-block sums lj, clmb, wall, vir[3] (6 x nblk)
   double *red;      // block_sum scratch
   int *flag;        // barrier status broadcast (16 ints)
@@ -1336,9 +1337,8 @@ __host__ __device__ inline int resident_region0(int nmol, int nsite, int prec) {
 __host__ __device__ inline int resident_lds_bytes(int nmol, int nsite, int prec, int nt, bool site_dbg,
                                                   bool gm = false) {
   return (gm ? 0 : align16(NFIELD * 8 * nmol) + align16(4 * nmol)) + resident_region0(nmol, nsite, prec) +
-         align16(4 * nmol) + npart(nsite, site_dbg) * 8 * nt + align16(6 * 8 * ((nmol + EB - 1) / EB)) +
-         (nt / WAVE) * 32 * 8 + 64 +
-         128;
+         align16(4 * nmol) + npart(nsite, site_dbg) * 8 * nt +
+         (gm ? 0 : align16(6 * 8 * ((nmol + EB - 1) / EB))) + (nt / WAVE) * 32 * 8 + 64 + 128;
 }
 
 template <int NT, int NS>
@@ -1361,9 +1361,14 @@ __device__ inline LdsR carve_resident(char *smem, int N, int prec, bool site_dbg
   L.je = L.js + N;
   p += align16(4 * N);
   L.part = (double *)p;
+  L.part_n = npart(NS, site_dbg) * NT;
   p += npart(NS, site_dbg) * 8 * NT;
-  L.E = (double *)p;
-  p += align16(6 * 8 * ((N + EB - 1) / EB));
+  if (gmol) {
+    L.E = gmol + (size_t)NFIELD * N;  // block sums in the private global copy
+  } else {
+    L.E = (double *)p;
+    p += align16(6 * 8 * ((N + EB - 1) / EB));
+  }
   L.red = (double *)p;
   p += (NT / WAVE) * 32 * 8;
   L.flag = (int *)p;
@@ -1476,7 +1481,7 @@ __device__ void rank_sort_resident(JRef J, double *mol, int *mid, int N, double 
   __syncthreads();
   // permute the state through the larger of the pair-partial area (idle
   // during the sort) and the rest of region 0, as many fields per round as fit
-  const int cap_part = (int)((S.E - S.part) / N);  // S.part .. S.E is the partial area
+  const int cap_part = S.part_n / N;
   const int cap_r0 = (resident_region0(N, NS, J.precision) - align16(10 * N)) / (8 * N);
   double *tmp = cap_part >= cap_r0 ? S.part : (double *)(S.region0 + align16(10 * N));
   const int fit = cap_part >= cap_r0 ? cap_part : cap_r0;
@@ -1860,7 +1865,7 @@ __device__ __attribute__((always_inline)) void run_resident(const LaunchArgs &a,
   int *gmid = nullptr;
   if (GM) {
     gmol = gen(R.gbuf) + (size_t)g.part * gm_part_doubles(N);
-    gmid = (int *)(gmol + (size_t)NFIELD * N);
+    gmid = (int *)(gmol + (size_t)NFIELD * N + (size_t)6 * ((N + EB - 1) / EB));
   }
   LdsR S = carve_resident<NT, NS>(smem, N, J.precision, R.site_f != nullptr, gmol, gmid);
   double *mol = S.mol;
