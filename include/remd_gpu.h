@@ -67,8 +67,8 @@ typedef struct {
                             several workgroups), _SPLIT (as RESIDENT with the state in a
                             private global copy per workgroup, for replicas too large for
                             the LDS) or _GLOBAL (state in HBM, one workgroup per replica);
-                            AUTO takes RESIDENT when the replica fits, else SPLIT when it
-                            gets 4 or more workgroups alone, else GLOBAL */
+                            AUTO takes RESIDENT when the replica fits, else SPLIT when
+                            that fits, else GLOBAL */
 } remd_gpu_desc;
 
 enum { REMD_PATH_AUTO = 0, REMD_PATH_RESIDENT = 1, REMD_PATH_GLOBAL = 2, REMD_PATH_SPLIT = 3 };

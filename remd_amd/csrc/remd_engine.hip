@@ -544,19 +544,11 @@ extern "C" int remd_gpu_create(remd_gpu **out, const remd_gpu_desc *d, const rem
                                  std::to_string(lim) + " bytes of LDS, or block_threads != " +
                                  std::to_string(RNT) + ")");
   }
-  int per_cu_gm = 0, ncu_dev = 0;
-  if (d->path == REMD_PATH_AUTO && !fits && fits_gm) {
-    // AUTO takes the split-global variant for a replica too large for the
-    // LDS when it would get 4 or more workgroups alone; with fewer the
-    // global-state path's 512-thread workgroups are faster
-    int dev = 0;
-    hipGetDevice(&dev);
-    if (resident_batch_occupancy(glds, &per_cu_gm) != hipSuccess ||
-        hipDeviceGetAttribute(&ncu_dev, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-      per_cu_gm = ncu_dev = 0;
-  }
-  const bool auto_gm = per_cu_gm > 0 && ncu_dev > 0 && pick_parts(per_cu_gm * ncu_dev, g->nrep, g->N) >= 4;
-  g->gm = d->path == REMD_PATH_SPLIT || (d->path == REMD_PATH_AUTO && !fits && auto_gm);
+  // AUTO takes the split-global variant for a replica too large for the
+  // LDS: in a batch it joins the one resident launch (the global-state
+  // classes only start after it), and alone it splits over up to GMAX
+  // workgroups
+  g->gm = d->path == REMD_PATH_SPLIT || (d->path == REMD_PATH_AUTO && !fits && fits_gm);
   g->resident = d->path == REMD_PATH_RESIDENT || (d->path == REMD_PATH_AUTO && fits) || g->gm;
   if (g->resident) {
     g->nt = RNT;
