@@ -166,9 +166,10 @@ struct remd_batch {
   unsigned *d_blk = nullptr;
   int blk_cap = 0, nblk = 0;
   bool split = false;  // some replica of the plan has parts > 1
-  // autotuner: candidate plans (best estimate first), each timed on one
-  // interval after an untimed one; the fastest is kept until the batch
-  // changes.  Results do not depend on the plan.
+  // autotuner: candidate plans (the cost model's best estimates first, then
+  // hill climbing around the fastest timed plan), each timed on one interval
+  // after an untimed one; the fastest is kept until the batch changes.
+  // Results do not depend on the plan.
   std::vector<std::vector<int>> cands;
   std::vector<double> cand_ms;
   int cand_cur = 0;
@@ -1403,10 +1404,12 @@ static int rebuild_table(remd_batch *b) {
   if (n) HIPCHK(hipMemcpyAsync(b->d_table, flat.data(), sizeof(ReplicaDesc) * n, hipMemcpyHostToDevice, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
   b->ntab = n;
+  bool fixed = false;
   if (!b->forced.empty() && b->forced.size() == b->jobs.size()) {
     if (!layout_plan(b, b->forced, nullptr))
       return fail(REMD_EINVAL, "the fixed parts need more workgroups than the device holds at once");
     b->cands.assign(1, b->forced);
+    fixed = true;
   } else {
     b->forced.clear();
     make_candidates(b);
@@ -1414,11 +1417,42 @@ static int rebuild_table(remd_batch *b) {
   b->cand_ms.assign(b->cands.size(), 0.0);
   b->cand_cur = 0;
   b->warm = 1;
-  b->tuned = !b->autotune || b->cands.size() <= 1;
+  b->tuned = !b->autotune || fixed;
   int rc = apply_plan(b, b->cands[0]);
   if (rc) return rc;
   b->dirty = false;
   return 0;
+}
+
+// Hill climbing after the model's candidates: the plans one part away (for
+// one job, up or down) from the fastest plan timed so far, not yet timed
+// and fitting the device, best estimate first; at most 16 plans are timed
+// per batch composition.
+static void add_neighbours(remd_batch *b) {
+  constexpr size_t MAXTIMED = 16;
+  if (b->cands.size() >= MAXTIMED) return;
+  const size_t best = (size_t)(std::min_element(b->cand_ms.begin(), b->cand_ms.end()) - b->cand_ms.begin());
+  const std::vector<int> base = b->cands[best];
+  std::vector<std::pair<double, std::vector<int>>> nb;
+  for (size_t k = 0; k < b->jobs.size(); k++) {
+    const remd_gpu *g = b->jobs[k];
+    if (!g->resident) continue;
+    const int pmax = std::min(GMAX, std::max(1, g->N / 8));
+    for (int dp = -1; dp <= 1; dp += 2) {
+      std::vector<int> P = base;
+      P[k] += dp;
+      if (P[k] < 1 || P[k] > pmax) continue;
+      if (std::find(b->cands.begin(), b->cands.end(), P) != b->cands.end()) continue;
+      if (!layout_plan(b, P, nullptr)) continue;
+      nb.push_back({plan_time(b, P), P});
+    }
+  }
+  std::stable_sort(nb.begin(), nb.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+  for (const auto &c : nb) {
+    if (b->cands.size() >= MAXTIMED) break;
+    b->cands.push_back(c.second);
+    b->cand_ms.push_back(0.0);
+  }
 }
 
 extern "C" int remd_batch_execute(remd_batch *b) {
@@ -1500,6 +1534,7 @@ extern "C" int remd_batch_execute(remd_batch *b) {
       b->warm--;
     } else {
       b->cand_ms[b->cand_cur] = ms;
+      if (b->cand_cur + 1 >= (int)b->cands.size()) add_neighbours(b);
       if (b->cand_cur + 1 < (int)b->cands.size()) {
         b->cand_cur++;
       } else {

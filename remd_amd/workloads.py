@@ -98,12 +98,12 @@ def _timed_intervals(run_one, n):
     return time.perf_counter() - t0, lat
 
 
-def uneven_batch_study(intervals=10, warmup=6):
+def uneven_batch_study(intervals=10, warmup=16):
     """Eight nanotube systems (R = 5..12 A, Lz = 40 A, nmol ~40-420,
     TIP3P/TIP4P alternating, 32 replicas each = 256 replicas): all jobs in
     one batch (one resident launch per interval, replicas split over
     workgroups by the batch's packing plan; the warm-up intervals include
-    the autotuner's timed candidates) against the same jobs one at a time
+    the autotuner's timed plans) against the same jobs one at a time
     (each job's replicas in one launch)."""
     import numpy as np
 
