@@ -575,8 +575,9 @@ static void D6(orc_system *s) {
   }
 }
 
-/* D5 (src/integrator.cxx:564-601); the 2D branch is the area barostat of
- * the contract. */
+/* D5 (src/integrator.cxx:564-601); the 2D branch is the area barostat
+ * (one area momentum, Lx = Ly = sqrt(A)) of the reference's SQUARE CUDA
+ * branch (src/remdgpu.cu:2103-2107), not the CPU path's L[0]-only update. */
 static void D5(orc_system *s) {
   const double cv0 = s->L[0] * s->s * s->dthalf;
   const double cv1 = s->L[1] * s->s * s->dthalf;

@@ -9,8 +9,8 @@
  * build: SWITCHING + FOUR_SITE tuned kernel, nlane = 1, one OpenMP thread)
  * and cites the reference file:line it follows.  Where the reference has no
  * usable semantics (3/5-site site roles, slit wall, 2D/3D barostat factors)
- * the oracle implements the contract documented in DESIGN.md ("semantics
- * contract"); those places say so in remd_oracle.c.
+ * the oracle implements the same rule as the GPU kernels; those places say
+ * so, with the reason, in remd_oracle.c.
  *
  * Parity pins (tests/test_oracle_*.py):
  *   - bit-for-bit / 1e-13 agreement with oracle/_ref (the reference's own CPU
