@@ -254,7 +254,7 @@ def test_large_replica_takes_split_path():
     assert np.array_equal(sg["id"], so["id"])
     assert rel_rms(sg["f"], so["f"]) <= 1e-5 and rel_rms(sg["n"], so["n"]) <= 1e-5
     g.execute_steps(s.T, s.P)
-    assert g.launch_path()[1] >= 4
+    assert g.launch_path()[1] == 8
     h = ra.REMDGPU(s, path=ra.PATH_GLOBAL)
     h.execute_steps(s.T, s.P)
     for k in (0, s.nreplica - 1):
