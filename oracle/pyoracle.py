@@ -418,5 +418,35 @@ class RefSystem:
             pass
 
 
+def ref_parallel_worker(mtl_path, desc, reps, budget_s, barrier, out):
+    """One process of the replica-parallel CPU timing: the reference's FP32
+    units (one thread) run this process's share of the replicas, one
+    interval per call, round-robin until `budget_s` of timed work.  The
+    reference runs replicas one after another (the replica loop of
+    src/remd.cxx:257-267 has its OpenMP pragma commented out), so one
+    process per core is the multi-core figure.  `reps` holds
+    (r, v, q, p, ids, scal, T, P) per replica; puts (steps, seconds)."""
+    import time
+
+    lib = RefLib(True)
+    systems = []
+    for rep in reps:
+        rs = RefSystem(lib, mtl_path, desc, *rep)
+        rs.quiet(lib.L.ref_initialize_property)
+        systems.append(rs)
+    steps = int(desc["interval"])
+    done = 0
+    barrier.wait()
+    t0 = time.perf_counter()
+    while True:
+        for rs in systems:
+            rs.quiet(lib.L.ref_execute_steps)
+            done += steps
+        elapsed = time.perf_counter() - t0
+        if elapsed > budget_s:
+            break
+    out.put((done, elapsed))
+
+
 def ref_available() -> bool:
     return os.path.exists(REF_F32) and os.path.exists(REF_F64)
