@@ -8,7 +8,10 @@ Tolerances (stated per quantity):
   * energies at step 0: |dE| <= 1e-5 * sum|E| (mixed), 1e-10 (FP64);
   * one 100-step interval from the reference fixture: FP64 pair math
     <= 1e-7 relative on positions / momenta (chaotic growth of 1e-16
-    summation-order differences), mixed <= 1e-3.
+    summation-order differences), mixed <= 1e-3;
+  * one 20-step interval of each barostat mode (none, 1D, 2D area, 3D
+    isotropic) against the FP64 oracle: FP64 pair math <= 1e-9, mixed
+    <= 1e-4 on positions, momenta and the thermostat / barostat scalars.
 """
 import numpy as np
 import pytest
@@ -177,6 +180,63 @@ def test_interval_reference_fixture(golden, prec, tol, path):
     for k, idx in (("lj", 0), ("clmb", 1), ("wall", 2), ("tot", 11)):
         print(k, av[idx], pr[k])
     assert abs(av[11] - pr["tot"]) <= 1e3 * tol * abs(pr["tot"]) + 1e-6
+    g.close()
+
+
+def _barostat_case(conf):
+    from helpers import relax_oracle
+
+    if conf == "tube":
+        s, pmode = inp.config_tube(seed=31, nmol=96, Lz=36.0, ntemp=2, npress=1, nfwall=500), 2
+    elif conf == "slit":
+        s, pmode = inp.config_slit(seed=32, nmol=100, Lxy=24.0, ntemp=2, npress=1, model="TIP5P"), 3
+    elif conf == "bulk":
+        s, pmode = inp.config_bulk(seed=33, ntemp=2, npress=1, model="TIP3P"), 1
+    else:  # "bulk-np": thermostat only
+        s, pmode = inp.config_bulk(seed=34, ntemp=2, npress=1, model="TIP4P"), 0
+    s = relax_oracle(s, steps=100, dt_fs=0.5)
+    s.tconstant, s.pconstant, s.dt_fs, s.interval = 1, pmode, 0.5, 20
+    s.P = np.full(s.nreplica, 50e6 / inp.UNIT_PRESS)
+    return s
+
+
+@pytest.mark.parametrize("path", PATHS)
+@pytest.mark.parametrize("conf", ["tube", "slit", "bulk", "bulk-np"])
+@pytest.mark.parametrize("prec,tol", [(1, 1e-9), (0, 1e-4)])
+def test_interval_vs_oracle_all_barostats(conf, prec, tol, path):
+    """One 20-step interval (0.5 fs) of the NP thermostat with the 1D (tube,
+    pconstant 2), 2D area (TIP5P slit, pconstant 3) and 3D isotropic (TIP3P
+    bulk, 256 molecules, pconstant 1) Andersen barostats, and with no
+    barostat, against the FP64 oracle at 50 MPa: positions (periodic difference of fractions),
+    velocities, quaternions, angular momenta, s and the box to `tol`
+    relative; Ps and Pv to `tol` of max(|value|, 1e-3).  The systems are
+    relaxed on the oracle first (100 steps at 0.5 fs, velocity scaling).
+    The oracle's own FP32 build departs from its FP64 build by about 1e-6
+    on these runs, so 1e-4 leaves room only for real differences."""
+    s = _barostat_case(conf)
+    g = ra.REMDGPU(s, precision=prec, path=path)
+    g.execute_steps(s.T, s.P)
+    for rep in range(s.nreplica):
+        o = oracle_for(s, rep)
+        o.initialize_property()
+        o.execute_steps(s.interval)
+        so, sg = o.state(), g.state(rep)
+        sc = o.scalars()
+        dd = by_id(sg["r"], sg["id"]) - by_id(so["r"], so["id"])
+        dd -= np.rint(dd)
+        errs = {"r": float(np.max(np.abs(dd)))}
+        for name in ("v", "q", "p"):
+            errs[name] = rel_rms(by_id(sg[name], sg["id"]), by_id(so[name], so["id"]))
+        gs = sg["scal"]
+        errs["s"] = abs(gs[0] - sc["s"]) / abs(sc["s"])
+        errs["L"] = float(np.max(np.abs(gs[7:10] - np.array(sc["L"])) / np.abs(np.array(sc["L"]))))
+        errs["Ps"] = abs(gs[1] - sc["Ps"]) / max(abs(sc["Ps"]), 1e-3)
+        errs["Pv"] = float(np.max(np.abs(gs[3:6] - np.array(sc["Pv"])))) / max(np.max(np.abs(sc["Pv"])), 1e-3)
+        print(conf, prec, rep, {k: f"{e:.1e}" for k, e in errs.items()}, "L", sc["L"], "Pv", sc["Pv"])
+        for k, e in errs.items():
+            assert e <= tol, (k, e)
+        if s.pconstant != 0:
+            assert not np.allclose(sc["L"], s.states[rep].L, rtol=1e-12, atol=0.0), "the box did not move"
     g.close()
 
 
