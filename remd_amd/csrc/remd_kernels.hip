@@ -1269,6 +1269,8 @@ __device__ void run_replica(const LaunchArgs &a, const DescD &R, JRef J, char *s
     c.P = gen(R.sc)[S_PNEW];
     c.H0 = total_energy<NT>(J, mol, N, c, S.red);
   }
+  // the driver scales once per interval before ExecuteSteps
+  // (src/remd.cxx:259-262), which scales again at every step
   if (J.tmode == 2) velocity_scaling<NT>(J, mol, N, c, S.red);
   for (int st = 0; st < J.interval; ++st) {
     if (J.tmode == 2) velocity_scaling<NT>(J, mol, N, c, S.red);
@@ -1932,6 +1934,8 @@ __device__ __attribute__((always_inline)) void run_resident(const LaunchArgs &a,
       c.P = Pn;
       c.H0 = total_energy<NT>(J, mol, N, c, S.red);
     }
+    // the driver scales once per interval before ExecuteSteps
+    // (src/remd.cxx:259-262), which scales again at every step
     if (J.tmode == 2) velocity_scaling<NT>(J, mol, N, c, S.red);
     double *pt = lead ? S.ptimer : nullptr;
     (void)pt;

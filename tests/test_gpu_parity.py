@@ -240,6 +240,60 @@ def test_interval_vs_oracle_all_barostats(conf, prec, tol, path):
     g.close()
 
 
+@pytest.mark.parametrize("path", PATHS)
+@pytest.mark.parametrize("conf,tmode", [("tube", 1), ("slit", 1), ("bulk", 1), ("tube", 2), ("bulk-np", 2)])
+def test_exchanged_conditions_vs_oracle(conf, tmode, path):
+    """Two intervals with new conditions between them, as after an accepted
+    exchange (src/remd.cxx:526-536: ChangeTemperature, KillMomentum,
+    SetPressure, CalcProperties, UpdateHamiltonian): replica 0 gets only a
+    new pressure (ChangeTemperature at the same T still folds s into the
+    momenta and resets s to 1), replica 1 only a new temperature.  With
+    tconstant 1 (Nose-Poincare) and 2 (velocity scaling once per interval
+    in the driver, src/remd.cxx:259-262, and at every step inside
+    ExecuteSteps), FP64 pair math against the FP64 oracle to 1e-9."""
+    s = _barostat_case(conf)
+    s.tconstant = tmode
+    T1 = s.T.copy()
+    P1 = s.P.copy()
+    T2 = np.array([T1[0], T1[0]])
+    P2 = np.array([2.0 * P1[0], P1[1]])
+    g = ra.REMDGPU(s, precision=1, path=path)
+    g.execute_steps(T1, P1)
+    g.execute_steps(T2, P2)
+    for rep in range(s.nreplica):
+        o = oracle_for(s, rep)
+        o.initialize_property()
+        for T_, P_ in ((None, None), (T2[rep], P2[rep])):
+            if T_ is not None:
+                o.change_temperature(float(T_))
+                o.kill_momentum()
+                o.sys.P = float(P_)
+                o.calc_properties()
+                o.update_hamiltonian()
+            if tmode == 2:  # once per interval in the driver, src/remd.cxx:259-262
+                o.velocity_scaling()
+                o.ang_velocity_scaling()
+            o.execute_steps(s.interval)
+        so, sg = o.state(), g.state(rep)
+        sc = o.scalars()
+        dd = by_id(sg["r"], sg["id"]) - by_id(so["r"], so["id"])
+        dd -= np.rint(dd)
+        errs = {"r": float(np.max(np.abs(dd)))}
+        for name in ("v", "p"):
+            errs[name] = rel_rms(by_id(sg[name], sg["id"]), by_id(so[name], so["id"]))
+        gs = sg["scal"]
+        errs["s"] = abs(gs[0] - sc["s"]) / abs(sc["s"])
+        errs["L"] = float(np.max(np.abs(gs[7:10] - np.array(sc["L"])) / np.abs(np.array(sc["L"]))))
+        av = g.averages(rep)
+        oa = o.average()
+        errs["tot"] = abs(av[11] - oa[11]) / max(abs(oa[11]), 1.0)
+        errs["ham"] = abs(av[12] - oa[12]) / max(abs(oa[11]), 1.0)
+        print(conf, tmode, rep, {k: f"{e:.1e}" for k, e in errs.items()}, "T", av[6], "s", sc["s"])
+        for k, e in errs.items():
+            assert e <= 1e-9, (k, e)
+    g.close()
+
+
 # ------------------------------------------------------------------ e1
 def test_exchange_device_matches_oracle():
     sysm = inp.config_tube(seed=21, nmol=64, Lz=30.0, ntemp=4, npress=2, nfwall=500)
