@@ -294,6 +294,50 @@ def test_exchanged_conditions_vs_oracle(conf, tmode, path):
     g.close()
 
 
+@pytest.mark.parametrize("model", ["TIP4P", "TIP5P"])
+def test_write_cdv_snapshot(tmp_path, model):
+    """remd_gpu_write_cdv (REMDGPU::OutputCDV, src/remdgpu.cu:700-804): one
+    file per replica, <prefix>R<rep:04d>s<step:07d>.cdv, a header line with
+    the box and the replica's T (K) and P (MPa), then `mol site x y z` (A)
+    for every site.  Checked against the downloaded state with invariants
+    that do not depend on the rotation convention: the mass-weighted site
+    centre is the molecule's centre of mass, and every intra-molecular site
+    distance is the body-frame one (to the 6 significant digits of %g).
+    The system is equilibrated first so that it stays inside the box."""
+    from remd_amd.workloads import relax
+
+    s = relax(inp.config_tube(seed=12, nmol=60, Lz=30.0, ntemp=2, npress=1, nfwall=500, model=model))
+    g = ra.REMDGPU(s, precision=1)
+    g.execute_steps(s.T, s.P)
+    g.write_cdv(str(tmp_path / "snap"), 1234)
+    mt = inp.molecule(model)
+    ns = mt.nsite
+    for rep in range(s.nreplica):
+        lines = (tmp_path / f"snapR{rep:04d}s{1234:07d}.cdv").read_text().splitlines()
+        head = lines[0]
+        R = s.wall_length
+        st = g.state(rep)
+        L = st["scal"][7:10]
+        assert head.startswith("'")
+        assert f"box_sx={-R:g} box_sy={-R:g} box_sz={-0.5 * L[2]:g}" in head
+        assert f"T={s.T[rep] * inp.UNIT_TEMP:g} K" in head
+        assert f"P={s.P[rep] * inp.UNIT_PRESS / 1e6:g} MPa" in head
+        rows = np.array([[float(x) for x in ln.split()] for ln in lines[1:]])
+        assert rows.shape == (s.nmol * ns, 5)
+        assert np.array_equal(rows[:, 0], np.repeat(np.arange(s.nmol), ns))
+        assert np.array_equal(rows[:, 1], np.tile(np.arange(ns), s.nmol))
+        x = rows[:, 2:].reshape(s.nmol, ns, 3)
+        tol = 4 * 5e-6 * max(1.0, float(np.max(np.abs(x))))  # 6 significant digits
+        com = (st["r"] - 0.5) * L
+        centre = np.einsum("a,nad->nd", mt.m, x) / mt.m.sum()
+        assert np.max(np.abs(centre - com)) < tol
+        for a in range(ns):
+            for b in range(a + 1, ns):
+                d = np.linalg.norm(x[:, a] - x[:, b], axis=1)
+                assert np.max(np.abs(d - np.linalg.norm(mt.r[a] - mt.r[b]))) < tol, (a, b, tol)
+    g.close()
+
+
 # ------------------------------------------------------------------ e1
 def test_exchange_device_matches_oracle():
     sysm = inp.config_tube(seed=21, nmol=64, Lz=30.0, ntemp=4, npress=2, nfwall=500)
