@@ -725,12 +725,12 @@ struct PairK {
   int per[3];
 };
 
-template <int NS>
+template <int NS, typename AT = double>
 struct Acc {
-  double site[NS][3];
-  double corr[3];
-  double vir[3];
-  double lj, clmb;
+  AT site[NS][3];
+  AT corr[3];
+  AT vir[3];
+  AT lj, clmb;
 };
 
 __device__ inline float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
@@ -743,9 +743,9 @@ __device__ inline double rnd(double x) { return rint(x); }
 // One ordered molecule pair (i, j): switched LJ (site 0 - site 0) + Coulomb
 // over charged site pairs, CPU tuned-kernel semantics
 // (src/force_calculator.cxx:874-989).  Pair math in PT, accumulation in FP64.
-template <int NS, int QB, typename PT>
+template <int NS, int QB, typename PT, typename AT>
 __device__ __forceinline__ void pair_acc(const PT *gi, const PT (&ai)[NS][3], const PT *rj,
-                                         const PairK<NS, PT> &k, Acc<NS> &acc) {
+                                         const PairK<NS, PT> &k, Acc<NS, AT> &acc) {
   PT dg[3];
 #pragma unroll
   for (int d = 0; d < 3; d++) {
@@ -801,24 +801,24 @@ __device__ __forceinline__ void pair_acc(const PT *gi, const PT (&ai)[NS][3], co
   }
   const PT cr = -dswr * (elj + ecl);
   PT F0 = cr * dg[0], F1 = cr * dg[1], F2 = cr * dg[2];
-  acc.corr[0] += (double)F0;
-  acc.corr[1] += (double)F1;
-  acc.corr[2] += (double)F2;
+  acc.corr[0] += (AT)F0;
+  acc.corr[1] += (AT)F1;
+  acc.corr[2] += (AT)F2;
 #pragma unroll
   for (int a = 0; a < NS; a++) {
     const PT sx = sw * fs[a][0], sy = sw * fs[a][1], sz = sw * fs[a][2];
-    acc.site[a][0] += (double)sx;
-    acc.site[a][1] += (double)sy;
-    acc.site[a][2] += (double)sz;
+    acc.site[a][0] += (AT)sx;
+    acc.site[a][1] += (AT)sy;
+    acc.site[a][2] += (AT)sz;
     F0 += sx;
     F1 += sy;
     F2 += sz;
   }
-  acc.vir[0] += (double)(F0 * dg[0]);
-  acc.vir[1] += (double)(F1 * dg[1]);
-  acc.vir[2] += (double)(F2 * dg[2]);
-  acc.lj += (double)(sw * elj);
-  acc.clmb += (double)(sw * ecl);
+  acc.vir[0] += (AT)(F0 * dg[0]);
+  acc.vir[1] += (AT)(F1 * dg[1]);
+  acc.vir[2] += (AT)(F2 * dg[2]);
+  acc.lj += (AT)(sw * elj);
+  acc.clmb += (AT)(sw * ecl);
 }
 
 // The partners of work item (i, c): jj = js + c, js + c + C, ... <= je
@@ -828,10 +828,10 @@ __device__ __forceinline__ void pair_acc(const PT *gi, const PT (&ai)[NS][3], co
 // still in order: the same pairs and the same sums, with less divergence
 // between the lanes of a wave where most partners lie beyond the cutoff
 // (bulk and slit windows).
-template <int NS, int QB, typename PT>
-__device__ __forceinline__ void pair_window(const PT *rec, int N, int i, int js, int je, int c0, int C,
-                                            bool compact, const PT *gi, const PT (&ai)[NS][3],
-                                            const PairK<NS, PT> &k, Acc<NS> &acc) {
+template <int NS, int QB, typename PT, typename AT>
+__device__ __forceinline__ void pair_window_run(const PT *rec, int N, int i, int js, int je, int c0, int C,
+                                                bool compact, const PT *gi, const PT (&ai)[NS][3],
+                                                const PairK<NS, PT> &k, Acc<NS, AT> &acc) {
   constexpr int ST = Rec<NS>::STRIDE;
   if (!compact) {
     for (int jj = js + c0; jj <= je; jj += C) {
@@ -873,6 +873,39 @@ __device__ __forceinline__ void pair_window(const PT *rec, int N, int i, int js,
       for (int v = 0; v < 3 + 3 * NS; v++) rj[v] = src[v];
       pair_acc<NS, QB, PT>(gi, ai, rj, k, acc);
     }
+  }
+}
+
+// One work item's partner sums into `acc` (zero on entry).  FP32 pair math
+// sums its chunk (about window/C partners) in FP32 and converts once; FP64
+// pair math sums in FP64.  The chunks are then reduced in FP64.  A chunk's
+// partners do not depend on how a replica is split over workgroups, so the
+// results stay the same bits for any split.
+template <int NS, int QB, typename PT>
+__device__ __forceinline__ void pair_window(const PT *rec, int N, int i, int js, int je, int c0, int C,
+                                            bool compact, const PT *gi, const PT (&ai)[NS][3],
+                                            const PairK<NS, PT> &k, Acc<NS> &acc) {
+  if constexpr (sizeof(PT) == 4) {
+    Acc<NS, float> part;
+#pragma unroll
+    for (int a = 0; a < NS; a++) part.site[a][0] = part.site[a][1] = part.site[a][2] = 0.0f;
+    part.corr[0] = part.corr[1] = part.corr[2] = 0.0f;
+    part.vir[0] = part.vir[1] = part.vir[2] = 0.0f;
+    part.lj = part.clmb = 0.0f;
+    pair_window_run<NS, QB, PT>(rec, N, i, js, je, c0, C, compact, gi, ai, k, part);
+#pragma unroll
+    for (int a = 0; a < NS; a++)
+#pragma unroll
+      for (int d = 0; d < 3; d++) acc.site[a][d] = (double)part.site[a][d];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      acc.corr[d] = (double)part.corr[d];
+      acc.vir[d] = (double)part.vir[d];
+    }
+    acc.lj = (double)part.lj;
+    acc.clmb = (double)part.clmb;
+  } else {
+    pair_window_run<NS, QB, PT>(rec, N, i, js, je, c0, C, compact, gi, ai, k, acc);
   }
 }
 
