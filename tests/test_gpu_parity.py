@@ -496,18 +496,74 @@ def test_mixed_path_batch_equals_alone():
     gb.close()
 
 
+@pytest.mark.skip(reason="batched FP64-pair jobs can differ from the same job alone after new conditions "
+                         "(TIP3P); remd_batch_add refuses FP64-pair jobs until the cause is found")
+def test_mixed_precision_batch_equals_alone():
+    """FP32-pair and FP64-pair jobs of three models in one batch (the
+    packed launch's single- and double-precision variants side by side,
+    resident and split-global) give, state and averages, the bits each job
+    gives alone, over two intervals."""
+    from remd_amd.workloads import relax
+
+    specs = [("TIP4P", 0, ra.PATH_RESIDENT), ("TIP3P", 1, ra.PATH_RESIDENT), ("TIP5P", 1, ra.PATH_SPLIT),
+             ("TIP4P", 1, ra.PATH_RESIDENT)]
+    systems = []
+    for k, (model, prec, path) in enumerate(specs):
+        s = relax(inp.config_tube(seed=41 + k, nmol=56 + 8 * k, Lz=24.0 + 2 * k, ntemp=2, npress=1,
+                                  nfwall=500, model=model))
+        s.interval = 20
+        systems.append((s, prec, path))
+    alone = []
+    for s, prec, path in systems:
+        g = ra.REMDGPU(s, precision=prec, path=path)
+        g.execute_steps(s.T, s.P)
+        g.execute_steps(s.T[::-1].copy(), s.P)
+        alone.append((g.all_averages(), g.state(1)))
+        g.close()
+    jobs = [ra.REMDGPU(s, precision=prec, path=path) for s, prec, path in systems]
+    bt = ra.Batch()
+    for g in jobs:
+        bt.add(g)
+    for T_of in (lambda s: s.T, lambda s: s.T[::-1].copy()):
+        for g, (s, _, _) in zip(jobs, systems):
+            g.set_conditions(T_of(s), s.P)
+        bt.execute()
+    for g, (av, st) in zip(jobs, alone):
+        assert np.array_equal(g.all_averages(), av)
+        s1 = g.state(1)
+        for name in ("r", "v", "q", "p", "id"):
+            assert np.array_equal(s1[name], st[name]), name
+    bt.close()
+    for g in jobs:
+        g.close()
+
+
+def test_batch_refuses_fp64_jobs():
+    """remd_batch_add rejects a job with FP64 pair math (it runs alone) and
+    keeps batching FP32-pair jobs."""
+    s = inp.config_tube(seed=45, nmol=48, Lz=24.0, ntemp=2, npress=1, nfwall=500)
+    g64, g32 = ra.REMDGPU(s, precision=1), ra.REMDGPU(s, precision=0)
+    bt = ra.Batch()
+    with pytest.raises(RuntimeError, match="FP64-pair jobs are not batched"):
+        bt.add(g64)
+    bt.add(g32)
+    bt.close()
+    g64.close()
+    g32.close()
+
+
 def test_uneven_autotuned_batch_equals_alone():
-    """Four jobs of different sizes, models, confinements and pair
-    precisions in one batch for six intervals -- the autotuner times
-    different packing plans on the first ones -- give the bits each gives
-    alone."""
+    """Four jobs of different sizes, models and confinements in one batch
+    for six intervals -- the autotuner times different packing plans on
+    the first ones -- give the bits each gives alone (FP32 pair math: the
+    batch refuses FP64-pair jobs)."""
     from remd_amd.workloads import relax
 
     jobs = [inp.config_tube(seed=51, nmol=40, Lz=22.0, ntemp=4, npress=1, model="TIP3P", nfwall=400),
             inp.config_tube(seed=52, nmol=90, Lz=30.0, ntemp=4, npress=2, nfwall=400),
             inp.config_slit(seed=53, nmol=120, ntemp=2, npress=2, model="TIP5P", Lxy=30.0),
             inp.config_bulk(seed=54, nmol=64, ntemp=2, model="TIP4P")]
-    precs = [0, 0, 0, 1]
+    precs = [0, 0, 0, 0]
     jobs = [relax(s) for s in jobs]  # random placements, equilibrated at 0.5 fs
     for s in jobs:
         s.interval = 20
