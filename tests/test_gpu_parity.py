@@ -538,6 +538,36 @@ def test_mixed_precision_batch_equals_alone():
         g.close()
 
 
+@pytest.mark.parametrize("model,nmol", [("TIP3P", 64), ("TIP4P", 56), ("TIP5P", 64)])
+def test_batched_job_with_new_conditions_equals_alone(model, nmol):
+    """A batched FP32-pair job whose temperatures are swapped before its
+    second interval (the exchange rescale runs inside the launch) gives
+    the bits of the same job alone, for 1, 2 and 4 workgroups per replica
+    and the autotuned plan."""
+    from remd_amd.workloads import relax
+
+    s = relax(inp.config_tube(seed=42, nmol=nmol, Lz=26.0, ntemp=2, npress=1, nfwall=500, model=model))
+    s.interval = 20
+    conds = (s.T.copy(), s.T[::-1].copy())
+    g = ra.REMDGPU(s, precision=0, path=ra.PATH_RESIDENT)
+    for T in conds:
+        g.execute_steps(T, s.P)
+    ref = g.all_averages()
+    g.close()
+    for parts in (None, [1], [2], [4]):
+        g = ra.REMDGPU(s, precision=0, path=ra.PATH_RESIDENT)
+        bt = ra.Batch()
+        bt.add(g)
+        if parts:
+            bt.set_parts(parts)
+        for T in conds:
+            g.set_conditions(T, s.P)
+            bt.execute()
+        assert np.array_equal(g.all_averages(), ref), (model, parts, bt.parts())
+        bt.close()
+        g.close()
+
+
 def test_batch_refuses_fp64_jobs():
     """remd_batch_add rejects a job with FP64 pair math (it runs alone) and
     keeps batching FP32-pair jobs."""
