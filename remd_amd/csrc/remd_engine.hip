@@ -1170,11 +1170,6 @@ extern "C" int remd_batch_create(remd_batch **out) {
 
 extern "C" int remd_batch_add(remd_batch *b, remd_gpu *g) {
   if (!b || !g) return fail(REMD_EINVAL, "null argument");
-  // A batched FP64-pair job can differ from the same job run alone after
-  // new conditions (seen for TIP3P); until the cause is found such jobs run
-  // alone through remd_gpu_execute_steps.
-  if (g->prm.precision != 0)
-    return fail(REMD_EINVAL, "FP64-pair jobs are not batched; run them with remd_gpu_execute_steps");
   if (std::find(b->jobs.begin(), b->jobs.end(), g) != b->jobs.end()) return 0;
   b->jobs.push_back(g);
   b->dirty = true;

@@ -496,8 +496,6 @@ def test_mixed_path_batch_equals_alone():
     gb.close()
 
 
-@pytest.mark.skip(reason="batched FP64-pair jobs can differ from the same job alone after new conditions "
-                         "(TIP3P); remd_batch_add refuses FP64-pair jobs until the cause is found")
 def test_mixed_precision_batch_equals_alone():
     """FP32-pair and FP64-pair jobs of three models in one batch (the
     packed launch's single- and double-precision variants side by side,
@@ -529,6 +527,7 @@ def test_mixed_precision_batch_equals_alone():
             g.set_conditions(T_of(s), s.P)
         bt.execute()
     for g, (av, st) in zip(jobs, alone):
+        assert np.all(np.isfinite(av[:, :15]))
         assert np.array_equal(g.all_averages(), av)
         s1 = g.state(1)
         for name in ("r", "v", "q", "p", "id"):
@@ -553,6 +552,8 @@ def test_batched_job_with_new_conditions_equals_alone(model, nmol):
     for T in conds:
         g.execute_steps(T, s.P)
     ref = g.all_averages()
+    ref_states = [g.state(r) for r in range(s.nreplica)]
+    assert np.all(np.isfinite(ref[:, :15]))
     g.close()
     for parts in (None, [1], [2], [4]):
         g = ra.REMDGPU(s, precision=0, path=ra.PATH_RESIDENT)
@@ -564,36 +565,26 @@ def test_batched_job_with_new_conditions_equals_alone(model, nmol):
             g.set_conditions(T, s.P)
             bt.execute()
         assert np.array_equal(g.all_averages(), ref), (model, parts, bt.parts())
+        for r in range(s.nreplica):
+            st = g.state(r)
+            for name in ("r", "v", "q", "p", "id"):
+                assert np.array_equal(st[name], ref_states[r][name]), (model, parts, r, name)
         bt.close()
         g.close()
-
-
-def test_batch_refuses_fp64_jobs():
-    """remd_batch_add rejects a job with FP64 pair math (it runs alone) and
-    keeps batching FP32-pair jobs."""
-    s = inp.config_tube(seed=45, nmol=48, Lz=24.0, ntemp=2, npress=1, nfwall=500)
-    g64, g32 = ra.REMDGPU(s, precision=1), ra.REMDGPU(s, precision=0)
-    bt = ra.Batch()
-    with pytest.raises(RuntimeError, match="FP64-pair jobs are not batched"):
-        bt.add(g64)
-    bt.add(g32)
-    bt.close()
-    g64.close()
-    g32.close()
 
 
 def test_uneven_autotuned_batch_equals_alone():
     """Four jobs of different sizes, models and confinements in one batch
     for six intervals -- the autotuner times different packing plans on
-    the first ones -- give the bits each gives alone (FP32 pair math: the
-    batch refuses FP64-pair jobs)."""
+    the first ones -- give the bits each gives alone; one job has FP64
+    pair math."""
     from remd_amd.workloads import relax
 
     jobs = [inp.config_tube(seed=51, nmol=40, Lz=22.0, ntemp=4, npress=1, model="TIP3P", nfwall=400),
             inp.config_tube(seed=52, nmol=90, Lz=30.0, ntemp=4, npress=2, nfwall=400),
             inp.config_slit(seed=53, nmol=120, ntemp=2, npress=2, model="TIP5P", Lxy=30.0),
             inp.config_bulk(seed=54, nmol=64, ntemp=2, model="TIP4P")]
-    precs = [0, 0, 0, 0]
+    precs = [0, 1, 0, 0]
     jobs = [relax(s) for s in jobs]  # random placements, equilibrated at 0.5 fs
     for s in jobs:
         s.interval = 20
