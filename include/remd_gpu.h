@@ -171,8 +171,7 @@ int remd_gpu_execute_pending(remd_gpu *g);
 /* ---- batching: several jobs, one launch per interval ---- */
 typedef struct remd_batch remd_batch;
 int remd_batch_create(remd_batch **out);
-/* Adds a job (REMD_EINVAL for a job with FP64 pair math: such jobs run
- * alone through remd_gpu_execute_steps). */
+/* Adds a job (any model, path and pair precision). */
 int remd_batch_add(remd_batch *b, remd_gpu *g);
 int remd_batch_remove(remd_batch *b, remd_gpu *g);
 /* Stage conditions for a job's next batched interval. */
