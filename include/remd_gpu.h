@@ -101,6 +101,17 @@ int remd_gpu_execute_steps(remd_gpu *g, const double *T, const double *P);
  * plain one-replica-per-launch form used as the batching baseline). */
 int remd_gpu_execute_steps_plain(remd_gpu *g, const double *T, const double *P);
 
+/* As remd_gpu_execute_steps, allocating nothing (no counterpart in the
+ * reference; for callers that cannot allocate device memory: code under
+ * HIP graph capture, tight serving loops).  The launch's scratch -- group
+ * and abort words, the replica table, the split-global block table and the
+ * exchange rows of replicas split over workgroups -- lives in one static
+ * device array of 16 MiB per GPU (held until the process ends); a job that
+ * needs more gets REMD_EINVAL.  The array keeps the last call's contents
+ * and every call on one GPU shares it: calls that overlap on one GPU race.
+ * Same results, bit for bit, as remd_gpu_execute_steps. */
+int remd_gpu_execute_steps_noalloc(remd_gpu *g, const double *T, const double *P);
+
 /* REMDGPU::GetAverages (src/remdgpu.h:208-233) in the Average enum order of
  * src/remdinfo.h:27-41: LJ CLMB WALL VOL TRA ROT T P Px Py Pz TOT DRF TSTs
  * TSTv BSTx BSTy BSTz DUMMY (T in K, pressures in Pa, as the CPU path). */

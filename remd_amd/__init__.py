@@ -70,7 +70,8 @@ def lib():
     L.remd_gpu_last_error.restype = C.c_char_p
     L.remd_gpu_create.argtypes = [C.POINTER(C.c_void_p), C.POINTER(Desc), C.POINTER(Scalars), _dp, _dp, _dp,
                                   _dp, _ip]
-    for n in ("remd_gpu_execute_steps", "remd_gpu_execute_steps_plain", "remd_gpu_set_conditions"):
+    for n in ("remd_gpu_execute_steps", "remd_gpu_execute_steps_plain", "remd_gpu_execute_steps_noalloc",
+              "remd_gpu_set_conditions"):
         getattr(L, n).argtypes = [C.c_void_p, _dp, _dp]
     L.remd_gpu_execute_pending.argtypes = [C.c_void_p]
     L.remd_gpu_get_averages.argtypes = [C.c_void_p, C.c_int, _dp]
@@ -210,6 +211,13 @@ class REMDGPU:
         P = np.ascontiguousarray(P, np.float64)
         fn = lib().remd_gpu_execute_steps_plain if plain else lib().remd_gpu_execute_steps
         _check(fn(self.h, _dp_of(T), _dp_of(P)))
+
+    def execute_steps_noalloc(self, T, P):
+        """remd_gpu_execute_steps_noalloc: the same interval with the launch
+        scratch in the library's static device array (no allocation)."""
+        T = np.ascontiguousarray(T, np.float64)
+        P = np.ascontiguousarray(P, np.float64)
+        _check(lib().remd_gpu_execute_steps_noalloc(self.h, _dp_of(T), _dp_of(P)))
 
     def set_conditions(self, T, P):
         T = np.ascontiguousarray(T, np.float64)

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -39,6 +40,10 @@ struct ExchangeJob {
   int *accepted;
 };
 hipError_t launch_exchange(const ExchangeJob *jobs, int njobs, hipStream_t st);
+hipError_t launch_scratch_setup(const ReplicaDesc *src, ReplicaDesc *dst, int nrep, double *xbase,
+                                size_t xstride, unsigned long long *sync, int nsync, unsigned *blk, int nb,
+                                int parts, unsigned variant, hipStream_t st);
+int resident_grid(int nrep, int parts);
 int lds_bytes_host(int nmol, int nsite, int prec, int nt);
 }  // namespace remd
 
@@ -104,6 +109,7 @@ constexpr int GMAX = 8;
 struct remd_gpu {
   remd_gpu_desc desc;
   JobParams prm;
+  int device = 0;            // HIP device the job was created on
   int nrep = 0, N = 0, NS = 0;
   int nt = 256, lds = 0;
   bool resident = false;     // state in LDS for the launch (resident_kernel)
@@ -597,6 +603,7 @@ extern "C" int remd_gpu_create(remd_gpu **out, const remd_gpu_desc *d, const rem
     }                                  \
   } while (0)
   CK([&]() -> int {
+    HIPCHK(hipGetDevice(&g->device));
     HIPCHK(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&g->ev0));
     HIPCHK(hipEventCreate(&g->ev1));
@@ -691,6 +698,102 @@ extern "C" int remd_gpu_execute_steps(remd_gpu *g, const double *T, const double
   if (rc) return rc;
   rc = launch(g, MODE_INTERVAL, 0, 0.0, g->d_reps, g->nrep, true);
   if (rc) return rc;
+  rc = sync_scalars(g);
+  if (rc) return rc;
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, g->ev0, g->ev1));
+  g->last_ms = ms;
+  return 0;
+}
+
+// ---------------------------------------------- allocation-free interval
+// remd_gpu_execute_steps_noalloc keeps the scratch of its launch -- group
+// words and abort word, the replica table, the split-global block table and
+// the per-step exchange rows of replicas split over workgroups -- in this
+// file-scope device array (one copy per GPU, held until the process ends),
+// so the call allocates nothing.  Host code reaches the array only through
+// the address hipGetSymbolAddress returns for the device (kept per device
+// below); the array's name is not a device address on the host.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)16 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+constexpr int MAXDEV = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_dev[MAXDEV];
+
+// Device address of this device's scratch array, looked up once per device
+// with that device current.
+static int scratch_address(int dev, void **out) {
+  if (dev < 0 || dev >= MAXDEV) return fail(REMD_EINVAL, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    int cur = 0;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != dev) HIPCHK(hipSetDevice(dev));
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) HIPCHK(hipSetDevice(cur));
+    if (e != hipSuccess || !p)
+      return fail(REMD_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return 0;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+extern "C" int remd_gpu_execute_steps_noalloc(remd_gpu *g, const double *T, const double *P) {
+  if (!g) return fail(REMD_EINVAL, "null handle");
+  const int parts = g->resident ? pick_parts(g->capacity, g->nrep, g->N) : 1;
+  const int nsync = g->nrep + 1;
+  const int nb = g->gm ? resident_grid(g->nrep, parts) : 0;
+  const size_t xstride = 2 * xbuf_parity_doubles(g->N);
+  const size_t off_tab = align256(sizeof(unsigned long long) * nsync);
+  const size_t off_blk = off_tab + align256(sizeof(ReplicaDesc) * g->nrep);
+  const size_t off_x = off_blk + align256(sizeof(unsigned) * nb);
+  const size_t need = off_x + (parts > 1 ? sizeof(double) * xstride * g->nrep : 0);
+  if (need > NOALLOC_SCRATCH_BYTES)
+    return fail(REMD_EINVAL, "the job needs " + std::to_string(need) + " bytes of launch scratch; the static "
+                             "scratch of remd_gpu_execute_steps_noalloc holds " +
+                             std::to_string(NOALLOC_SCRATCH_BYTES) + " (use remd_gpu_execute_steps)");
+  void *base = nullptr;
+  int rc = scratch_address(g->device, &base);
+  if (rc) return rc;
+  rc = remd_gpu_set_conditions(g, T, P);
+  if (rc) return rc;
+  char *sb = (char *)base;
+  unsigned long long *sync = (unsigned long long *)sb;
+  ReplicaDesc *tab = (ReplicaDesc *)(sb + off_tab);
+  unsigned *blk = (unsigned *)(sb + off_blk);
+  double *xb = parts > 1 ? (double *)(sb + off_x) : nullptr;
+  const unsigned variant = (unsigned)(6 + (g->NS - 3) * 2 + g->prm.precision);
+  HIPCHK(launch_scratch_setup(g->d_reps, tab, g->nrep, xb, xstride, sync, nsync, nb ? blk : nullptr, nb, parts,
+                              variant, g->stream));
+  LaunchArgs a;
+  a.reps = tab;
+  a.nrep = g->nrep;
+  a.mode = MODE_INTERVAL;
+  a.do_sort = 0;
+  a.margin = 0.0;
+  a.parts = parts;
+  a.same_xcd_ok = same_xcd_barrier_enabled();
+  a.same_xcd_mode = same_xcd_barrier_mode();
+  a.sync = sync;
+  a.abort = (unsigned *)(sync + g->nrep);
+  g->last_parts = parts;
+  HIPCHK(hipEventRecord(g->ev0, g->stream));
+  if (g->gm) {
+    a.blk = blk;
+    a.nblk = nb;
+    HIPCHK(launch_resident_batch(a, g->lds, g->stream));
+  } else if (g->resident) {
+    HIPCHK(launch_resident(a, g->NS, g->prm.precision, g->lds, g->stream));
+  } else {
+    HIPCHK(launch_interval(a, g->nt, g->NS, g->prm.precision, g->lds, g->stream));
+  }
+  HIPCHK(hipEventRecord(g->ev1, g->stream));
+  if (parts > 1)
+    HIPCHK(hipMemcpyAsync(g->h_abort, a.abort, sizeof(unsigned), hipMemcpyDeviceToHost, g->stream));
   rc = sync_scalars(g);
   if (rc) return rc;
   float ms = 0.f;

@@ -2254,6 +2254,37 @@ __global__ void exchange_kernel(const ExchangeJob *jobs, int njobs) {
   }
 }
 
+// ------------------------------------------------ scratch-table set-up
+// Set-up of an interval launch whose scratch sits in one caller-free buffer
+// (the allocation-free entry point): zeroes the nsync group words (the
+// abort word is the last), copies the job's replica table with replica k's
+// exchange rows moved to xbase + k * xstride (none when xbase is null), and
+// writes the block table of a split-global launch with the single-job
+// mapping (the parts of a replica on block indices equal mod 8).
+__global__ void scratch_setup_kernel(const ReplicaDesc *src, ReplicaDesc *dst, int nrep, double *xbase,
+                                     size_t xstride, unsigned long long *sync, int nsync, unsigned *blk,
+                                     int nb, int parts, unsigned variant) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nsync) sync[k] = 0ull;
+  if (k < nrep) {
+    ReplicaDesc d = src[k];
+    d.xbuf = xbase ? xbase + (size_t)k * xstride : nullptr;
+    dst[k] = d;
+  }
+  if (k < nb) {
+    int rep, part;
+    if (parts <= 1) {
+      rep = k;
+      part = 0;
+    } else {
+      const int slot = k >> 3;
+      rep = (slot / parts) * 8 + (k & 7);
+      part = slot % parts;
+    }
+    blk[k] = rep < nrep ? ((unsigned)rep << 12 | variant << 8 | (unsigned)parts << 4 | (unsigned)part) : ~0u;
+  }
+}
+
 // ------------------------------------------------------------ launchers
 // Kernel class of a replica: (block threads, sites, pair precision).
 template <int NT>
@@ -2386,6 +2417,17 @@ hipError_t launch_sort_window(const double *keys, int nrow, int n, double w, int
                               hipStream_t st) {
   const int lds = 20 * n + 8 + (16 * 64 + 16) * 4;
   hipLaunchKernelGGL(sort_window_kernel<256>, dim3(nrow), dim3(256), lds, st, keys, n, w, order, js, je);
+  return hipGetLastError();
+}
+
+hipError_t launch_scratch_setup(const ReplicaDesc *src, ReplicaDesc *dst, int nrep, double *xbase,
+                                size_t xstride, unsigned long long *sync, int nsync, unsigned *blk, int nb,
+                                int parts, unsigned variant, hipStream_t st) {
+  int n = nsync > nrep ? nsync : nrep;
+  if (nb > n) n = nb;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(scratch_setup_kernel, dim3((n + 255) / 256), dim3(256), 0, st, src, dst, nrep, xbase, xstride,
+                     sync, nsync, blk, nb, parts, variant);
   return hipGetLastError();
 }
 
