@@ -742,26 +742,34 @@ __device__ inline double rnd(double x) { return rint(x); }
 
 // One ordered molecule pair (i, j): switched LJ (site 0 - site 0) + Coulomb
 // over charged site pairs, CPU tuned-kernel semantics
-// (src/force_calculator.cxx:874-989).  Pair math in PT, accumulation in FP64.
+// (src/force_calculator.cxx:874-989).  Pair math in PT, accumulation in AT.
+// Branch-free: a pair at or beyond the cutoff gets switching factors 0 (as
+// the CPU kernel's r02 = min(rc^2, r02) does, :883), so every sum receives
+// a signed zero and keeps its bits; `self` (j == i) moves the partner far
+// outside the cutoff first, so that every term stays finite.
 template <int NS, int QB, typename PT, typename AT>
 __device__ __forceinline__ void pair_acc(const PT *gi, const PT (&ai)[NS][3], const PT *rj,
-                                         const PairK<NS, PT> &k, Acc<NS, AT> &acc) {
+                                         const PairK<NS, PT> &k, bool self, Acc<NS, AT> &acc) {
   PT dg[3];
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     dg[d] = gi[d] - rj[d];
     if (k.per[d]) dg[d] -= k.L[d] * rnd(dg[d] * k.Li[d]);
   }
+  dg[2] = self ? PT(1.0e4) : dg[2];
   const PT r2 = dg[0] * dg[0] + dg[1] * dg[1] + dg[2] * dg[2];
-  if (!(r2 < k.rc2)) return;
-  PT sw = PT(1), dswr = PT(0);
-  if (r2 > k.rl2) {
+  const bool inside = r2 < k.rc2;
+  PT sw, dswr;
+  {
     const PT ri = rsq(r2);
     const PT r = r2 * ri;
     const PT c_ = r - k.rc;
     const PT l_ = r - k.rl;
-    sw = k.coef * c_ * c_ * c_ * (PT(10) * l_ * l_ - PT(5) * l_ * c_ + c_ * c_);
-    dswr = k.coef30 * c_ * c_ * l_ * l_ * ri;
+    const PT s5 = k.coef * c_ * c_ * c_ * (PT(10) * l_ * l_ - PT(5) * l_ * c_ + c_ * c_);
+    const PT d5 = k.coef30 * c_ * c_ * l_ * l_ * ri;
+    const bool switched = r2 > k.rl2;
+    sw = inside ? (switched ? s5 : PT(1)) : PT(0);
+    dswr = (inside && switched) ? d5 : PT(0);
   }
   PT di[NS][3];
 #pragma unroll
@@ -769,8 +777,6 @@ __device__ __forceinline__ void pair_acc(const PT *gi, const PT (&ai)[NS][3], co
 #pragma unroll
     for (int d = 0; d < 3; d++) di[a][d] = dg[d] + ai[a][d];
   PT fs[NS][3];
-#pragma unroll
-  for (int a = 0; a < NS; a++) fs[a][0] = fs[a][1] = fs[a][2] = PT(0);
   PT elj, ecl = PT(0);
   {  // LJ between the two site-0 (oxygen) sites
     const PT x = di[0][0] - rj[3], y = di[0][1] - rj[4], z = di[0][2] - rj[5];
@@ -780,10 +786,12 @@ __device__ __forceinline__ void pair_acc(const PT *gi, const PT (&ai)[NS][3], co
     const PT s6 = s2 * s2 * s2;
     elj = k.eps4 * s6 * (s6 - PT(1));
     const PT f = k.eps48 * s6 * (s6 - PT(0.5)) * ri2;
-    fs[0][0] += f * x;
-    fs[0][1] += f * y;
-    fs[0][2] += f * z;
+    fs[0][0] = f * x;
+    fs[0][1] = f * y;
+    fs[0][2] = f * z;
   }
+#pragma unroll
+  for (int a = 1; a < NS; a++) fs[a][0] = fs[a][1] = fs[a][2] = PT(0);
 #pragma unroll
   for (int a = QB; a < NS; a++) {
 #pragma unroll
@@ -821,57 +829,95 @@ __device__ __forceinline__ void pair_acc(const PT *gi, const PT (&ai)[NS][3], co
   acc.clmb += (AT)(sw * ecl);
 }
 
+template <int NS, typename PT>
+__device__ __forceinline__ void load_rec(PT (&r)[3 + 3 * NS], const PT *src) {
+#pragma unroll
+  for (int u = 0; u < 3 + 3 * NS; u++) r[u] = src[u];
+}
+
 // The partners of work item (i, c): jj = js + c, js + c + C, ... <= je
-// (indices mod N), each through pair_acc, in this order.  With `compact`
-// the COM distances of the next 32 partners are screened first (radius
-// slightly above the cutoff) and only the ones inside go through pair_acc,
-// still in order: the same pairs and the same sums, with less divergence
-// between the lanes of a wave where most partners lie beyond the cutoff
-// (bulk and slit windows).
+// (indices mod N), each through pair_acc, in this order.  The record of the
+// next partner is loaded while the current pair is computed.  With
+// `compact` the COM distances of the next 32 partners are screened first
+// (radius slightly above the cutoff) and only the ones inside go through
+// pair_acc, still in order: the same pairs and the same sums, with less
+// work where most partners lie beyond the cutoff (bulk and slit windows).
 template <int NS, int QB, typename PT, typename AT>
 __device__ __forceinline__ void pair_window_run(const PT *rec, int N, int i, int js, int je, int c0, int C,
                                                 bool compact, const PT *gi, const PT (&ai)[NS][3],
                                                 const PairK<NS, PT> &k, Acc<NS, AT> &acc) {
   constexpr int ST = Rec<NS>::STRIDE;
+  constexpr int RL = 3 + 3 * NS;
   if (!compact) {
-    for (int jj = js + c0; jj <= je; jj += C) {
-      const int j = (jj >= N) ? jj - N : jj;
-      if (j == i) continue;
-      PT rj[3 + 3 * NS];
-      const PT *src = rec + j * ST;
+    int jj = js + c0;
+    if (jj > je) return;
+    int j = (jj >= N) ? jj - N : jj;
+    PT rj[RL];
+    load_rec<NS, PT>(rj, rec + j * ST);
+    for (;;) {
+      const int jn = jj + C;
+      const bool more = jn <= je;
+      const int jx = more ? ((jn >= N) ? jn - N : jn) : j;
+      PT rn[RL];
+      load_rec<NS, PT>(rn, rec + jx * ST);
+      pair_acc<NS, QB, PT>(gi, ai, rj, k, j == i, acc);
+      if (!more) break;
+      jj = jn;
+      j = jx;
 #pragma unroll
-      for (int u = 0; u < 3 + 3 * NS; u++) rj[u] = src[u];
-      pair_acc<NS, QB, PT>(gi, ai, rj, k, acc);
+      for (int u = 0; u < RL; u++) rj[u] = rn[u];
     }
     return;
   }
   for (int base = js + c0; base <= je; base += 32 * C) {
     unsigned mask = 0u;
-    for (int u = 0; u < 32; u++) {
-      const int jj = base + u * C;
-      if (jj > je) break;
-      const int j = (jj >= N) ? jj - N : jj;
-      if (j == i) continue;
-      const PT *src = rec + j * ST;
-      PT dg[3];
+    {
+      int jj = base;
+      int j = (jj >= N) ? jj - N : jj;
+      PT c[3] = {rec[j * ST], rec[j * ST + 1], rec[j * ST + 2]};
+      for (int u = 0; u < 32; u++) {
+        const int jn = jj + C;
+        const bool more = u < 31 && jn <= je;
+        const int jx = more ? ((jn >= N) ? jn - N : jn) : j;
+        const PT cn[3] = {rec[jx * ST], rec[jx * ST + 1], rec[jx * ST + 2]};
+        PT dg[3];
 #pragma unroll
-      for (int d = 0; d < 3; d++) {
-        dg[d] = gi[d] - src[d];
-        if (k.per[d]) dg[d] -= k.L[d] * rnd(dg[d] * k.Li[d]);
+        for (int d = 0; d < 3; d++) {
+          dg[d] = gi[d] - c[d];
+          if (k.per[d]) dg[d] -= k.L[d] * rnd(dg[d] * k.Li[d]);
+        }
+        const PT r2 = dg[0] * dg[0] + dg[1] * dg[1] + dg[2] * dg[2];
+        if (j != i && r2 < k.rs2) mask |= 1u << u;
+        if (!more) break;
+        jj = jn;
+        j = jx;
+        c[0] = cn[0];
+        c[1] = cn[1];
+        c[2] = cn[2];
       }
-      const PT r2 = dg[0] * dg[0] + dg[1] * dg[1] + dg[2] * dg[2];
-      if (r2 < k.rs2) mask |= 1u << u;
     }
-    while (mask) {
-      const int u = __builtin_ctz(mask);
-      mask &= mask - 1u;
-      const int jj = base + u * C;
-      const int j = (jj >= N) ? jj - N : jj;
-      PT rj[3 + 3 * NS];
-      const PT *src = rec + j * ST;
+    if (!mask) continue;
+    int u = __builtin_ctz(mask);
+    mask &= mask - 1u;
+    int j = base + u * C;
+    j = (j >= N) ? j - N : j;
+    PT rj[RL];
+    load_rec<NS, PT>(rj, rec + j * ST);
+    for (;;) {
+      const bool more = mask != 0u;
+      int jx = j;
+      if (more) {
+        jx = base + __builtin_ctz(mask) * C;
+        jx = (jx >= N) ? jx - N : jx;
+        mask &= mask - 1u;
+      }
+      PT rn[RL];
+      load_rec<NS, PT>(rn, rec + jx * ST);
+      pair_acc<NS, QB, PT>(gi, ai, rj, k, false, acc);
+      if (!more) break;
+      j = jx;
 #pragma unroll
-      for (int v = 0; v < 3 + 3 * NS; v++) rj[v] = src[v];
-      pair_acc<NS, QB, PT>(gi, ai, rj, k, acc);
+      for (int v = 0; v < RL; v++) rj[v] = rn[v];
     }
   }
 }
