@@ -29,6 +29,9 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3  # MI355X dense FP32 vector peak
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_resident_kernel_tube144_tip4p_64rep.json")
+VALU_FILE = os.path.join(ROOT, "profiles", "valu_resident_kernel_tube144_tip4p_64rep.json")
+CPU_RATIO_FILE = os.path.join(ROOT, "profiles", "cpu_baseline_reference_vs_port.json")
+KERNEL_SRC = os.path.join(ROOT, "remd_amd", "csrc", "remd_kernels.hip")
 
 
 def measured_traffic(kernel):
@@ -48,93 +51,110 @@ def measured_traffic(kernel):
     return float((fetch + write) * kib)
 
 
-def _parallel_reference(system, T, P, mtl_path, budget_s):
-    """The reference's FP32 build on every host core this process may use
-    (at most 16): one spawned process per core, each on its share of the
-    replicas, started together; returns (replica-steps/s, processes)."""
-    import multiprocessing as mp
+def measured_valu(kernel, kernel_ms):
+    """Executed FP32 / FP64 VALU FLOP per launch of `kernel` from the
+    committed rocprofv3 counter run (SQ_INSTS_VALU_FLOPS_FP32 / _FP64 count
+    FLOP per wave instruction; x 64 lanes), as FLOP/s over this run's
+    kernel time, with the VALU-busy fraction of that run; None when no
+    measurement of this kernel is on file.  `stale` marks a counter run of
+    another kernel source than the one built here."""
+    if not os.path.exists(VALU_FILE):
+        return None
+    with open(VALU_FILE) as f:
+        m = json.load(f)
+    if m.get("kernel") != kernel:
+        return None
+    import hashlib
 
-    import pyoracle as po
+    with open(KERNEL_SRC, "rb") as f:
+        src = hashlib.sha256(f.read()).hexdigest()[:16]
+    fp32 = float(np.mean(m["flops_fp32_per_launch"]))
+    fp64 = float(np.mean(m["flops_fp64_per_launch"]))
+    return {"executed_fp32_tflops": round(fp32 / (kernel_ms * 1e-3) / 1e12, 4),
+            "executed_fp64_tflops": round(fp64 / (kernel_ms * 1e-3) / 1e12, 4),
+            "executed_fp32_flop_per_launch": fp32, "valu_busy": m["valu_busy"],
+            "lds_bank_conflict_cycles_per_lds_inst": m["lds_bank_conflict_per_lds_inst"],
+            "counters": os.path.relpath(VALU_FILE, ROOT), "stale": src != m.get("kernel_source_sha16")}
+
+
+def _parallel(worker, args_of, system, T, P, budget_s):
+    """One spawned process per host core this process may use (at most
+    16), each on its share of the replicas, started together; returns
+    (replica-steps/s, processes).  Every worker is joined, and any that has
+    not exited is terminated, before this returns."""
+    import multiprocessing as mp
 
     nproc = min(16, len(os.sched_getaffinity(0)), system.nreplica)
     ctx = mp.get_context("spawn")
     barrier = ctx.Barrier(nproc)
     out = ctx.Queue()
-    desc = system.desc()
     procs = []
-    for w in range(nproc):
-        reps = []
-        for k in range(w, system.nreplica, nproc):
-            st = system.states[k]
-            reps.append((st.r, st.v, st.q, st.p, st.ids, st.scal(), float(T[k]), float(P[k])))
-        pr = ctx.Process(target=po.ref_parallel_worker, args=(mtl_path, desc, reps, budget_s, barrier, out))
-        pr.start()
-        procs.append(pr)
-    res = [out.get(timeout=budget_s + 120) for _ in procs]
-    for pr in procs:
-        pr.join(timeout=30)
+    try:
+        for w in range(nproc):
+            reps = []
+            for k in range(w, system.nreplica, nproc):
+                st = system.states[k]
+                reps.append((st.r, st.v, st.q, st.p, st.ids, st.scal(), float(T[k]), float(P[k])))
+            pr = ctx.Process(target=worker, args=args_of(reps, barrier, out))
+            pr.start()
+            procs.append(pr)
+        res = [out.get(timeout=budget_s + 120) for _ in procs]
+    finally:
+        for pr in procs:
+            pr.join(timeout=30)
+        for pr in procs:
+            if pr.is_alive():
+                pr.terminate()
+                pr.join(timeout=10)
+        out.close()
+        out.join_thread()
     steps = sum(r[0] for r in res)
     return steps / max(r[1] for r in res), nproc
 
 
 def cpu_baseline(system, T, P, budget_s=12.0):
-    """Time the reference's own FP32 CPU build (oracle/_ref, makefile.gnu
-    flags) on the same relaxed replicas: one thread, one interval per
-    replica (the shipped configuration), then one process per host core
-    (at most 16) on shares of the replicas, which is `value`.  Falls back
-    to the oracle's FP32 port, one thread, where _ref was not built."""
+    """Time the oracle's FP32 restatement of the reference's CPU path (bit
+    for bit the reference's makefile.gnu FP32 build on these inputs) on the
+    same relaxed replicas: one thread, one interval per replica (the
+    shipped configuration), then one process per host core (at most 16)
+    on shares of the replicas, which is `value`.  The reference's own
+    build against this restatement, both timed in one container on the same
+    replicas, is CPU_RATIO_FILE."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import tempfile
-
     import pyoracle as po
     from remd_amd import inputs as inp
 
-    kind = "reference" if po.ref_available() else "port"
     nrep = system.nreplica
     steps = system.interval
     done = 0
     elapsed = 0.0
     mt_text = inp.MOLTYPE[system.model]
-    with tempfile.TemporaryDirectory() as td:
-        mtl = os.path.join(td, "moltype.txt")
-        with open(mtl, "w") as f:
-            f.write(mt_text)
-        fw = None if kind == "reference" else po.wall_table_tube(system.nfwall, system.wall_length,
-                                                                system.sgm_wall, system.eps_wall,
-                                                                system.rho_wall)
-        reflib = po.RefLib(True) if kind == "reference" else None
-        for k in range(nrep):
-            st = system.states[k]
-            desc = system.desc()
-            if kind == "reference":
-                rs = po.RefSystem(reflib, mtl, desc, st.r, st.v, st.q, st.p, st.ids, st.scal(), T[k], P[k])
-                rs.quiet(reflib.L.ref_initialize_property)
-                t0 = time.perf_counter()
-                rs.quiet(reflib.L.ref_execute_steps)
-                elapsed += time.perf_counter() - t0
-                del rs
-            else:
-                o = po.OracleSystem(desc, po.parse_moltype(mt_text), st.r, st.v, st.q, st.p, st.scal(), T[k], P[k],
-                                    fp32=True, fwall=fw)
-                o.initialize_property()
-                t0 = time.perf_counter()
-                o.execute_steps(steps)
-                elapsed += time.perf_counter() - t0
-            done += 1
-            if elapsed > budget_s:
-                break
-        single = done * steps / elapsed
-        if kind == "reference":
-            value, cores = _parallel_reference(system, T, P, mtl, budget_s)
-    if kind == "reference":
-        sample = (f"{nrep} replicas, makefile.gnu FP32 build: one thread on {done} replicas x 1 interval "
-                  f"({steps} steps); then {cores} processes (one per core) on shares of the {nrep} replicas, "
-                  f"whole intervals round-robin for about {budget_s:.0f} s")
-    else:
-        value, cores = single, 1
-        sample = f"{done} of {nrep} replicas x 1 interval ({steps} steps), oracle FP32 port, 1 thread"
-    return dict(value=value, unit="replica-steps/s", cores=cores, kind=kind, sample=sample,
-                single_thread_value=single, seconds=round(elapsed, 3))
+    fw = po.wall_table_tube(system.nfwall, system.wall_length, system.sgm_wall, system.eps_wall, system.rho_wall)
+    desc = system.desc()
+    mt = po.parse_moltype(mt_text)
+    for k in range(nrep):
+        st = system.states[k]
+        o = po.OracleSystem(desc, mt, st.r, st.v, st.q, st.p, st.scal(), T[k], P[k], fp32=True, fwall=fw)
+        o.initialize_property()
+        t0 = time.perf_counter()
+        o.execute_steps(steps)
+        elapsed += time.perf_counter() - t0
+        done += 1
+        if elapsed > budget_s:
+            break
+    single = done * steps / elapsed
+    value, cores = _parallel(po.port_parallel_worker,
+                             lambda reps, barrier, out: (mt_text, desc, reps, fw, budget_s, barrier, out),
+                             system, T, P, budget_s)
+    sample = (f"{nrep} replicas, oracle FP32 restatement (-O2): one thread on {done} replicas x 1 interval "
+              f"({steps} steps); then {cores} processes (one per core) on shares of the {nrep} replicas, "
+              f"whole intervals round-robin for about {budget_s:.0f} s")
+    out = dict(value=value, unit="replica-steps/s", cores=cores, kind="port", sample=sample,
+               single_thread_value=single, seconds=round(elapsed, 3))
+    if os.path.exists(CPU_RATIO_FILE):
+        with open(CPU_RATIO_FILE) as f:
+            out["reference_over_port_one_thread"] = json.load(f)["reference_over_port"]
+    return out
 
 
 def main():
@@ -144,7 +164,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--plain-steps", type=int, default=6)
     ap.add_argument("--workload", default="tube64", choices=["tube64", "uneven", "served", "precision"],
                     help="tube64: the headline line; the others print one line of their own study")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -281,6 +301,10 @@ def main():
             "kernel_ms": round(kernel_ms, 4),
         },
     }
+    valu = measured_valu(kname, kernel_ms)
+    if valu:
+        valu["executed_over_algorithmic"] = round(valu["executed_fp32_flop_per_launch"] / flops_per_launch, 3)
+        out["roofline"]["valu_counters"] = valu
     job.close()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(system, system.T, system.P)

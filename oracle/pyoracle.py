@@ -448,5 +448,31 @@ def ref_parallel_worker(mtl_path, desc, reps, budget_s, barrier, out):
     out.put((done, elapsed))
 
 
+def port_parallel_worker(mt_text, desc, reps, fwall, budget_s, barrier, out):
+    """As ref_parallel_worker, for the oracle's FP32 restatement of the same
+    CPU path (bit-exact to the reference's FP32 build): this process's share
+    of the replicas, one interval per call, round-robin until `budget_s`."""
+    import time
+
+    mt = parse_moltype(mt_text)
+    systems = []
+    for (r, v, q, p, ids, scal, T, P) in reps:
+        o = OracleSystem(desc, mt, r, v, q, p, scal, T, P, fp32=True, fwall=fwall)
+        o.initialize_property()
+        systems.append(o)
+    steps = int(desc["interval"])
+    done = 0
+    barrier.wait()
+    t0 = time.perf_counter()
+    while True:
+        for o in systems:
+            o.execute_steps(steps)
+            done += steps
+        elapsed = time.perf_counter() - t0
+        if elapsed > budget_s:
+            break
+    out.put((done, elapsed))
+
+
 def ref_available() -> bool:
     return os.path.exists(REF_F32) and os.path.exists(REF_F64)

@@ -220,19 +220,50 @@ __device__ inline void space_to_body(const double *a, const double *v, double *t
   t[2] = 2. * (a13 + a02) * v[0] + 2. * (a23 - a01) * v[1] + (a0sq - a1sq - a2sq + a3sq) * v[2];
 }
 
-// sin and cos of a rotation sub-step angle.  Below 0.125 rad the Taylor
-// series through x^11 (sin) and x^12 (cos) leaves a truncation error under
-// 1e-21 relative, below FP64 rounding; larger angles use sincos.
-__device__ inline void step_sincos(double x, double *sn, double *cs) {
-  if (fabs(x) < 0.125) {
-    const double x2 = x * x;
+// sin and cos of a rotation sub-step angle, from their Taylor series (no
+// library call, so the five rotations of a step stay short, branch-light
+// code).  |x| < 0.5 rad (every free rotation of an MD step at these
+// temperatures): sin through x^15, cos through x^16; truncation below
+// 1e-19 relative, under FP64 rounding of the result.  Larger angles:
+// Cody-Waite reduction by pi/2 with a three-part constant (exact products
+// for |k| < 2^20) to |r| <= pi/4, then sin through r^19 and cos through
+// r^20 (truncation below 1e-21) and the quadrant's sign and swap.
+__device__ inline void sincos_poly(double x, double *sn, double *cs, bool wide) {
+  const double x2 = x * x;
+  if (!wide) {
     *sn = x * (1.0 + x2 * (-1.0 / 6.0 + x2 * (1.0 / 120.0 + x2 * (-1.0 / 5040.0 + x2 * (1.0 / 362880.0 +
-                                                                   x2 * (-1.0 / 39916800.0))))));
-    *cs = 1.0 + x2 * (-0.5 + x2 * (1.0 / 24.0 + x2 * (-1.0 / 720.0 + x2 * (1.0 / 40320.0 +
-                                                                x2 * (-1.0 / 3628800.0 + x2 * (1.0 / 479001600.0))))));
+          x2 * (-1.0 / 39916800.0 + x2 * (1.0 / 6227020800.0 + x2 * (-1.0 / 1307674368000.0))))))));
+    *cs = 1.0 + x2 * (-0.5 + x2 * (1.0 / 24.0 + x2 * (-1.0 / 720.0 + x2 * (1.0 / 40320.0 + x2 * (-1.0 / 3628800.0 +
+          x2 * (1.0 / 479001600.0 + x2 * (-1.0 / 87178291200.0 + x2 * (1.0 / 20922789888000.0))))))));
   } else {
-    sincos(x, sn, cs);
+    *sn = x * (1.0 + x2 * (-1.0 / 6.0 + x2 * (1.0 / 120.0 + x2 * (-1.0 / 5040.0 + x2 * (1.0 / 362880.0 +
+          x2 * (-1.0 / 39916800.0 + x2 * (1.0 / 6227020800.0 + x2 * (-1.0 / 1307674368000.0 +
+          x2 * (1.0 / 355687428096000.0 + x2 * (-1.0 / 121645100408832000.0))))))))));
+    *cs = 1.0 + x2 * (-0.5 + x2 * (1.0 / 24.0 + x2 * (-1.0 / 720.0 + x2 * (1.0 / 40320.0 + x2 * (-1.0 / 3628800.0 +
+          x2 * (1.0 / 479001600.0 + x2 * (-1.0 / 87178291200.0 + x2 * (1.0 / 20922789888000.0 +
+          x2 * (-1.0 / 6402373705728000.0 + x2 * (1.0 / 2432902008176640000.0))))))))));
   }
+}
+
+__device__ inline void step_sincos(double x, double *sn, double *cs) {
+  if (fabs(x) < 0.5) {
+    sincos_poly(x, sn, cs, false);
+    return;
+  }
+  // pi/2 = P1 + P2 + P3: P1 and P2 carry 33 significant bits each, so k * P1
+  // and k * P2 are exact for |k| < 2^20
+  const double P1 = 1.5707963267341256;
+  const double P2 = 6.077100506303966e-11;
+  const double P3 = 2.0222662487959506e-21;
+  const double k = rint(x * 0.63661977236758134308);
+  const double r = ((x - k * P1) - k * P2) - k * P3;
+  double s, c;
+  sincos_poly(r, &s, &c, true);
+  const int q = (int)k & 3;
+  const double ss = (q & 1) ? c : s;
+  const double cc = (q & 1) ? s : c;
+  *sn = (q & 2) ? -ss : ss;
+  *cs = ((q + 1) & 2) ? -cc : cc;
 }
 
 // Free rotation about body axis k (src/integrator.cxx:467-563, P1..P3 of
