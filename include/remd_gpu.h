@@ -125,6 +125,19 @@ int remd_gpu_get_pot_vol(remd_gpu *g, double *pot, double *vol);
 int remd_gpu_download_state(remd_gpu *g, int rep, double *r, double *v, double *f, double *q,
                             double *p, double *n, int *ids, remd_gpu_scalars *sc);
 
+/* Checkpoint of the job's exact device state at an interval boundary --
+ * what REMDGPU::ConvertVariablesToHost (src/remdgpu.cu:806-854) hands to
+ * IOManager::WriteCheckPoint (src/io_manager.cxx:593-634), plus the forces,
+ * molecule ids, H0 and energies the text checkpoint drops -- as one opaque
+ * blob of remd_gpu_checkpoint_bytes(g) bytes.  Loading it into a job
+ * created with the same descriptor continues the run bit for bit (the
+ * reference re-derives forces and H0 from its text checkpoint instead,
+ * src/io_manager.cxx:528-591).  A blob of another shape, pair precision or
+ * chunk count is refused (REMD_EINVAL). */
+size_t remd_gpu_checkpoint_bytes(remd_gpu *g);
+int remd_gpu_checkpoint_save(remd_gpu *g, void *buf, size_t bytes);
+int remd_gpu_checkpoint_load(remd_gpu *g, const void *buf, size_t bytes);
+
 /* REMDGPU::OutputCDV (src/remdgpu.cu:700-804): one .cdv text file per
  * replica, <prefix>R<rep:04>s<step:07>.cdv. */
 int remd_gpu_write_cdv(remd_gpu *g, const char *prefix, unsigned step);
@@ -211,6 +224,12 @@ size_t remd_gpu_footprint(const remd_gpu_desc *desc);
 typedef struct { int r[34]; int k; } remd_rng;
 void remd_rng_seed(remd_rng *s, unsigned seed);
 int remd_rng_next(remd_rng *s);
+
+/* Exchange pair list `type` 0..3 (REMDInfo::ExchangeList,
+ * src/remdinfo.cxx:54-108) of an ntemp x npress grid: the condition slots
+ * a[k], b[k] of pair k in list order (either may be NULL); returns the
+ * number of pairs. */
+int remd_exchange_pairs(int ntemp, int npress, int type, int *a, int *b);
 
 /* REMD::ReplicaExchange (src/remd.cxx:486-538) on the host for one job:
  * pair list step%4 (src/remdinfo.cxx:54-108), Metropolis with one

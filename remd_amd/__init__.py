@@ -15,7 +15,7 @@ import numpy as np
 from . import inputs
 
 __all__ = ["REMDGPU", "Batch", "lib", "desc_for", "GlibcRand", "wall_table", "sort_window",
-           "AVERAGE_NAMES", "inputs", "exchange_host"]
+           "AVERAGE_NAMES", "inputs", "exchange_host", "exchange_pairs"]
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(PKG, "libremd_gpu.so")
@@ -79,6 +79,11 @@ def lib():
     L.remd_gpu_download_state.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _ip,
                                           C.POINTER(Scalars)]
     L.remd_gpu_write_cdv.argtypes = [C.c_void_p, C.c_char_p, C.c_uint]
+    L.remd_gpu_checkpoint_bytes.argtypes = [C.c_void_p]
+    L.remd_gpu_checkpoint_bytes.restype = C.c_size_t
+    L.remd_gpu_checkpoint_save.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+    L.remd_gpu_checkpoint_load.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+    L.remd_exchange_pairs.argtypes = [C.c_int, C.c_int, C.c_int, _ip, _ip]
     L.remd_gpu_destroy.argtypes = [C.c_void_p]
     L.remd_gpu_destroy.restype = None
     L.remd_gpu_compute_forces.argtypes = [C.c_void_p, C.c_int, C.c_double]
@@ -255,6 +260,18 @@ class REMDGPU:
                     scal=np.array([sc.s, sc.Ps, sc.Q, sc.Pv[0], sc.Pv[1], sc.Pv[2], sc.W, sc.L[0], sc.L[1],
                                    sc.L[2]]), T=sc.T, P=sc.P)
 
+    def checkpoint_save(self) -> bytes:
+        """remd_gpu_checkpoint_save: the job's exact device state."""
+        n = int(lib().remd_gpu_checkpoint_bytes(self.h))
+        buf = C.create_string_buffer(n)
+        _check(lib().remd_gpu_checkpoint_save(self.h, buf, n))
+        return buf.raw
+
+    def checkpoint_load(self, blob: bytes):
+        """remd_gpu_checkpoint_load: continue from a saved state."""
+        buf = C.create_string_buffer(blob, len(blob))
+        _check(lib().remd_gpu_checkpoint_load(self.h, buf, len(blob)))
+
     def write_cdv(self, prefix, step):
         _check(lib().remd_gpu_write_cdv(self.h, prefix.encode(), int(step)))
 
@@ -313,6 +330,17 @@ class REMDGPU:
         _check(lib().remd_gpu_exchange_device(self.h, int(ntemp), int(npress), int(step), _ip_of(index),
                                               _dp_of(T), _dp_of(P), _dp_of(u), _ip_of(acc), C.byref(npairs)))
         return index, T, P, acc[: npairs.value]
+
+
+def exchange_pairs(ntemp, npress, typ):
+    """remd_exchange_pairs: (a, b) condition slots of exchange list `typ`."""
+    n = ntemp * npress
+    a = np.zeros(max(1, n), np.int32)
+    b = np.zeros(max(1, n), np.int32)
+    k = lib().remd_exchange_pairs(int(ntemp), int(npress), int(typ), _ip_of(a), _ip_of(b))
+    if k < 0:
+        _check(k)
+    return a[:k].copy(), b[:k].copy()
 
 
 class Batch:

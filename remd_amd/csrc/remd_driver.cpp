@@ -278,6 +278,135 @@ void velocity_scaling(Replica &R, const MolType &mt, int confined, int n, double
   }
 }
 
+// Run state kept across a restart (the reference keeps none of it: its
+// step counter, exchange index, rand() stream and acceptance/tunnel counts
+// start afresh, src/remd.cxx:25-198): the next interval number, the
+// condition-slot -> replica index, the replica conditions, the exchange
+// stream, acceptance per pair of each exchange list
+// (REMDInfo::PairList, src/remdinfo.h:141-189), the tunnel counters
+// (src/remdinfo.h:191-218) and the job's exact device state
+// (remd_gpu_checkpoint_save).
+struct RunState {
+  long step = 0;
+  std::vector<int> index;
+  std::vector<double> T, P;
+  remd_rng rng;
+  std::vector<long> acc[4], rej[4];
+  std::vector<int> tc_count;
+  std::vector<unsigned char> tc_max, tc_min;
+};
+
+const char kRunMagic[8] = {'R', 'E', 'M', 'D', 'R', 'U', 'N', '1'};
+
+template <class T>
+void put(std::ofstream &os, const T *p, size_t n) {
+  os.write((const char *)p, sizeof(T) * n);
+}
+template <class T>
+void get(std::ifstream &is, T *p, size_t n, const std::string &path) {
+  is.read((char *)p, sizeof(T) * n);
+  if (!is) die(path + ": truncated run state");
+}
+
+void write_run_state(const std::string &path, const RunState &st, remd_gpu *g) {
+  const size_t nb = remd_gpu_checkpoint_bytes(g);
+  std::vector<char> blob(nb);
+  check(remd_gpu_checkpoint_save(g, blob.data(), nb), "checkpoint");
+  const std::string tmp = path + ".tmp";
+  {
+    std::ofstream os(tmp, std::ios::binary);
+    if (!os) die("open " + tmp + " failed");
+    os.write(kRunMagic, 8);
+    const int nrep = (int)st.index.size();
+    put(os, &st.step, 1);
+    put(os, &nrep, 1);
+    put(os, st.index.data(), nrep);
+    put(os, st.T.data(), nrep);
+    put(os, st.P.data(), nrep);
+    put(os, &st.rng, 1);
+    for (int t = 0; t < 4; t++) {
+      const int n = (int)st.acc[t].size();
+      put(os, &n, 1);
+      put(os, st.acc[t].data(), n);
+      put(os, st.rej[t].data(), n);
+    }
+    put(os, st.tc_count.data(), nrep);
+    put(os, st.tc_max.data(), nrep);
+    put(os, st.tc_min.data(), nrep);
+    put(os, &nb, 1);
+    os.write(blob.data(), (std::streamsize)nb);
+    if (!os) die("write " + tmp + " failed");
+  }
+  if (std::rename(tmp.c_str(), path.c_str()) != 0) die("rename " + tmp + " failed");
+}
+
+void read_run_state(const std::string &path, RunState &st, remd_gpu *g) {
+  std::ifstream is(path, std::ios::binary);
+  if (!is) die("open " + path + " failed");
+  char magic[8];
+  get(is, magic, 8, path);
+  if (std::memcmp(magic, kRunMagic, 8) != 0) die(path + ": not a remd_md run state");
+  int nrep = 0;
+  get(is, &st.step, 1, path);
+  get(is, &nrep, 1, path);
+  if (nrep != (int)st.index.size()) die(path + ": nreplica differs from the input");
+  get(is, st.index.data(), nrep, path);
+  get(is, st.T.data(), nrep, path);
+  get(is, st.P.data(), nrep, path);
+  get(is, &st.rng, 1, path);
+  for (int t = 0; t < 4; t++) {
+    int n = 0;
+    get(is, &n, 1, path);
+    if (n != (int)st.acc[t].size()) die(path + ": exchange lists differ from the input");
+    get(is, st.acc[t].data(), n, path);
+    get(is, st.rej[t].data(), n, path);
+  }
+  get(is, st.tc_count.data(), nrep, path);
+  get(is, st.tc_max.data(), nrep, path);
+  get(is, st.tc_min.data(), nrep, path);
+  size_t nb = 0;
+  get(is, &nb, 1, path);
+  std::vector<char> blob(nb);
+  get(is, blob.data(), nb, path);
+  check(remd_gpu_checkpoint_load(g, blob.data(), nb), "checkpoint load");
+}
+
+// REMDInfo::OutputAcceptRate (src/remdinfo.cxx:762-820): per pressure row
+// the T-even and T-odd pairs interleaved, then per temperature column the
+// P-even and P-odd pairs, "Ti Pi Tj Pj rate" with the conditions of the
+// pair's slots.
+void write_accept_rate(const std::string &path, const RunState &st, int dt, int dp,
+                       const std::vector<int> (&pa)[4], const std::vector<int> (&pb)[4]) {
+  FILE *fp = std::fopen(path.c_str(), "w");
+  if (!fp) die("open " + path + " failed");
+  std::fprintf(fp, "#Ti,Pi,Tj,Pj,AR\n");
+  auto line = [&](int type, int i) {
+    const int ind0 = st.index[pa[type][i]], ind1 = st.index[pb[type][i]];
+    const double rate = (double)st.acc[type][i] / (double)(st.acc[type][i] + st.rej[type][i]);
+    std::fprintf(fp, "%lf %lf %lf %lf %lf\n", st.T[ind0], st.P[ind0], st.T[ind1], st.P[ind1], rate);
+  };
+  const int ll[4] = {(int)pa[0].size(), (int)pa[1].size(), (int)pa[2].size(), (int)pa[3].size()};
+  for (int p = 0; p < dp; p++) {
+    const int i0 = p * ll[0] / dp, i2 = p * ll[2] / dp;
+    for (int i = 0; i < ll[0] / dp; i++) {
+      line(0, i + i0);
+      if (i >= ll[2] / dp) break;
+      line(2, i + i2);
+    }
+    std::fprintf(fp, "\n");
+  }
+  for (int t = 0; t < dt; t++) {
+    const int i1 = t * ll[1] / dt, i3 = t * ll[3] / dt;
+    for (int i = 0; i < ll[1] / dt; i++) {
+      line(1, i + i1);
+      if (i >= ll[3] / dt) break;
+      line(3, i + i3);
+    }
+    std::fprintf(fp, "\n");
+  }
+  std::fclose(fp);
+}
+
 std::string fmt_default(double x) {  // std::ostream default formatting, as in file names
   std::ostringstream s;
   s << x;
@@ -413,25 +542,51 @@ int main(int argc, char **argv) {
     }
   }
 
-  std::vector<int> index(nrep);
-  for (int i = 0; i < nrep; i++) index[i] = i;
-  std::vector<double> Trep(T), Prep(P);  // conditions indexed by replica
-  remd_rng rng;
-  remd_rng_seed(&rng, 1);  // rand() without srand()
+  RunState st;
+  st.index.resize(nrep);
+  for (int i = 0; i < nrep; i++) st.index[i] = i;
+  st.T = T;  // conditions indexed by replica
+  st.P = P;
+  remd_rng_seed(&st.rng, 1);  // rand() without srand()
+  std::vector<int> pa[4], pb[4];
+  for (int t = 0; t < 4; t++) {
+    const int n = remd_exchange_pairs(prm.ntemp, prm.npress, t, nullptr, nullptr);
+    check(n, "exchange list");
+    pa[t].resize(n);
+    pb[t].resize(n);
+    remd_exchange_pairs(prm.ntemp, prm.npress, t, pa[t].data(), pb[t].data());
+    st.acc[t].assign(n, 0);
+    st.rej[t].assign(n, 0);
+  }
+  st.tc_count.assign(nrep, 0);
+  st.tc_max.assign(nrep, 0);
+  st.tc_min.assign(nrep, 0);
+  // restart: the exact state of the previous run when it left one (written
+  // beside its checkpoints), else the text checkpoints alone, as the
+  // reference does
+  if (prm.restart == 1) {
+    const std::string f = prm.input_prefix + prm.chk_in + ".state";
+    if (std::ifstream(f).good()) {
+      read_run_state(f, st, g);
+      std::fprintf(stderr, "restart: continuing from interval %ld (%s)\n", st.step, f.c_str());
+    }
+  }
+  std::vector<int> &index = st.index;
+  std::vector<double> &Trep = st.T, &Prep = st.P;
   std::vector<double> pot(nrep), vol(nrep), ave(19 * (size_t)nrep);
   std::vector<int> accepted(nrep + 1);
-  std::vector<long> acc_count(nrep + 1, 0), try_count(nrep + 1, 0);
-  long step = 0;
   const long nint = prm.interval > 0 ? prm.ninterval : 0;
+  const long step0 = st.step;
   std::cout << std::setprecision(6);
   for (long s = 0; s < nint; ++s) {
+    const long step = st.step;
     check(remd_gpu_execute_steps(g, Trep.data(), Prep.data()), "ExecuteSteps");
     check(remd_gpu_get_pot_vol(g, pot.data(), vol.data()), "GetAverages");
     for (int rep = 0; rep < nrep; rep++) {
       double *a = &ave[19 * (size_t)rep];
       check(remd_gpu_get_averages(g, rep, a), "GetAverages");
-      std::cout << " rep " << rep << " " << s << " " << pot[rep] << " " << vol[rep] << " " << a[0] << " " << a[1]
-                << " " << a[2] << " " << a[6];
+      std::cout << " rep " << rep << " " << step << " " << pot[rep] << " " << vol[rep] << " " << a[0] << " "
+                << a[1] << " " << a[2] << " " << a[6];
       if (prm.confined == 1) std::cout << " " << a[10];
       else if (prm.confined == 2) std::cout << " " << a[8] << " " << a[9];
       std::cout << " " << a[4] << " " << a[5] << " " << a[13] << " " << a[14];
@@ -443,15 +598,25 @@ int main(int argc, char **argv) {
     if (prm.rem_type == 0) {
       int np = 0;
       check(remd_exchange_host(prm.ntemp, prm.npress, step, prm.pconstant, pot.data(), vol.data(), index.data(),
-                               Trep.data(), Prep.data(), nullptr, &rng, accepted.data(), &np),
+                               Trep.data(), Prep.data(), nullptr, &st.rng, accepted.data(), &np),
             "ReplicaExchange");
-      for (int k = 0; k < np; k++) {
-        try_count[k]++;
-        acc_count[k] += accepted[k];
-      }
+      const int type = (int)(step % 4);
+      for (int k = 0; k < np; k++) (accepted[k] ? st.acc[type][k] : st.rej[type][k])++;
+      // REMDInfo::CheckTunneling (src/remdinfo.h:389-392): the replica at the
+      // lowest slot reaches the minimum, the one at the highest the maximum
+      auto reach = [&](int rep, bool is_max) {
+        (is_max ? st.tc_max : st.tc_min)[rep] = 1;
+        if (st.tc_max[rep] && st.tc_min[rep]) {
+          st.tc_count[rep]++;
+          (is_max ? st.tc_min : st.tc_max)[rep] = 0;
+        }
+      };
+      reach(index[0], false);
+      reach(index[nrep - 1], true);
     }
     if (prm.snap_interval > 0 && step % prm.snap_interval == 0 && prm.cdv_out != "null")
       check(remd_gpu_write_cdv(g, (out + prm.cdv_out).c_str(), (unsigned)(step / prm.snap_interval)), "OutputCDV");
+    st.step = step + 1;
     const bool last = s + 1 == nint;
     if (prm.chk_out != "null" && ((prm.chk_interval > 0 && step % prm.chk_interval == 0) || last)) {
       std::vector<double> rr(3 * N), vv(3 * N), qq(4 * N), pq(4 * N);
@@ -465,24 +630,57 @@ int main(int argc, char **argv) {
         name << out << prm.chk_out << std::setw(4) << std::setfill('0') << index[rep] << ".chk";
         write_chk(name.str(), mt, rr, vv, qq, pq, ids, ss);
       }
+      write_run_state(out + prm.chk_out + ".state", st, g);
     }
-    step++;
   }
   for (FILE *f : logs) std::fclose(f);
   if (nint > 0) {
-    // index.dat (src/remd.cxx:816-836) and exchange acceptance per pair slot
+    const long step = st.step;
+    // index.dat (REMD::OutputBackUp, src/remd.cxx:816-836)
     FILE *fi = std::fopen((out + "/index.dat").c_str(), "w");
     if (fi) {
       for (int i = 0; i < nrep; i++) std::fprintf(fi, "%4d %4d\n", i, index[i]);
       std::fclose(fi);
     }
-    FILE *fa = std::fopen((out + "/accept_rate.dat").c_str(), "w");
-    if (fa) {
-      for (int k = 0; k <= nrep; k++)
-        if (try_count[k] > 0) std::fprintf(fa, "%4d %lf\n", k, (double)acc_count[k] / (double)try_count[k]);
-      std::fclose(fa);
+    // end-of-run outputs of src/remd.cxx:452-458
+    write_accept_rate(out + "/accept_rate.dat", st, prm.ntemp, prm.npress, pa, pb);
+    // REMDInfo::OutputTunnelCount (src/remdinfo.cxx:822-832)
+    FILE *ft = std::fopen((out + "/tunnel_count.dat").c_str(), "w");
+    if (!ft) die("open tunnel_count.dat failed");
+    for (int rep = 0; rep < nrep; rep++) std::fprintf(ft, " %d", st.tc_count[rep]);
+    std::fprintf(ft, "\n");
+    std::fclose(ft);
+    // REMD::OutputEnergy (src/remd.cxx:838-849): pot (+ P V) per replica
+    {
+      std::ofstream os(out + "/energy.dat");
+      if (!os) die("open energy.dat failed");
+      os << std::setw(8) << step;
+      for (int i = 0; i < nrep; i++) {
+        double e = pot[i];
+        if (prm.pconstant > 0) e += vol[i] * Prep[i];
+        os << " " << std::setw(16) << std::setprecision(16) << e;
+      }
+      os << std::endl;
+    }
+    // REMD::OutputIndex (src/remd.cxx:851-879): appended unless step is 0
+    {
+      const char *mode = step == 0 ? "w" : "a";
+      FILE *fpt = std::fopen((out + "/index_temp.dat").c_str(), mode);
+      FILE *fpp = std::fopen((out + "/index_press.dat").c_str(), mode);
+      if (!fpt || !fpp) die("open index_temp.dat / index_press.dat failed");
+      std::fprintf(fpt, " %7ld", step);
+      std::fprintf(fpp, " %7ld", step);
+      for (int i = 0; i < nrep; i++) {
+        std::fprintf(fpt, " %lf", Trep[i]);
+        std::fprintf(fpp, " %lf", Prep[i]);
+      }
+      std::fprintf(fpt, "\n");
+      std::fprintf(fpp, "\n");
+      std::fclose(fpt);
+      std::fclose(fpp);
     }
   }
+  (void)step0;
   remd_gpu_destroy(g);
   return 0;
 }

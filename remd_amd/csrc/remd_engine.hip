@@ -1091,6 +1091,81 @@ extern "C" int remd_gpu_download_state(remd_gpu *g, int rep, double *r, double *
   return 0;
 }
 
+// ------------------------------------------------------------ checkpoint
+// Exact device state of a job at an interval boundary: every replica's
+// molecule block (r, v, f, q, p, n in the internal representation of
+// REMDGPU::ConvertVariablesToHost, src/remdgpu.cu:806-854), molecule ids
+// and scalar block (thermostat, barostat, box, conditions, H0, energies,
+// virial).  Loading it into a job of the same shape continues the run bit
+// for bit: the next interval starts with a sort and uses the stored forces,
+// so nothing is recomputed (the reference's text checkpoint,
+// src/io_manager.cxx:528-634, re-derives forces and H0 at start-up).
+struct CkptHeader {
+  char magic[8];          // "REMDCKPT"
+  int version;            // 1
+  int nmol, nsite, nrep;
+  int nfield, nsc;
+  int precision, chunks;  // the launch shape that fixes the summation order
+  int pad;
+};
+
+extern "C" size_t remd_gpu_checkpoint_bytes(remd_gpu *g) {
+  if (!g) return 0;
+  const size_t N = g->N, R = g->nrep;
+  return sizeof(CkptHeader) + R * (NFIELD * N * sizeof(double) + NSC * sizeof(double) + N * sizeof(int));
+}
+
+extern "C" int remd_gpu_checkpoint_save(remd_gpu *g, void *buf, size_t bytes) {
+  if (!g || !buf) return fail(REMD_EINVAL, "null argument");
+  if (bytes < remd_gpu_checkpoint_bytes(g)) return fail(REMD_EINVAL, "checkpoint buffer too small");
+  const size_t N = g->N, R = g->nrep;
+  CkptHeader h;
+  std::memset(&h, 0, sizeof(h));
+  std::memcpy(h.magic, "REMDCKPT", 8);
+  h.version = 1;
+  h.nmol = g->N;
+  h.nsite = g->NS;
+  h.nrep = g->nrep;
+  h.nfield = NFIELD;
+  h.nsc = NSC;
+  h.precision = g->prm.precision;
+  h.chunks = g->prm.chunks;
+  char *o = (char *)buf;
+  std::memcpy(o, &h, sizeof(h));
+  o += sizeof(h);
+  HIPCHK(hipStreamSynchronize(g->stream));
+  HIPCHK(hipMemcpy(o, g->d_mol, sizeof(double) * NFIELD * N * R, hipMemcpyDeviceToHost));
+  o += sizeof(double) * NFIELD * N * R;
+  HIPCHK(hipMemcpy(o, g->d_sc, sizeof(double) * NSC * R, hipMemcpyDeviceToHost));
+  o += sizeof(double) * NSC * R;
+  HIPCHK(hipMemcpy(o, g->d_mid, sizeof(int) * N * R, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int remd_gpu_checkpoint_load(remd_gpu *g, const void *buf, size_t bytes) {
+  if (!g || !buf) return fail(REMD_EINVAL, "null argument");
+  if (bytes < sizeof(CkptHeader)) return fail(REMD_EINVAL, "checkpoint too short");
+  CkptHeader h;
+  std::memcpy(&h, buf, sizeof(h));
+  if (std::memcmp(h.magic, "REMDCKPT", 8) != 0 || h.version != 1)
+    return fail(REMD_EINVAL, "not a remd_gpu checkpoint (or another version)");
+  if (h.nmol != g->N || h.nsite != g->NS || h.nrep != g->nrep || h.nfield != NFIELD || h.nsc != NSC)
+    return fail(REMD_EINVAL, "checkpoint shape (nmol, nsite, nreplica) differs from the job's");
+  if (h.precision != g->prm.precision || h.chunks != g->prm.chunks)
+    return fail(REMD_EINVAL, "checkpoint of a job with another pair precision or chunk count");
+  if (bytes < remd_gpu_checkpoint_bytes(g)) return fail(REMD_EINVAL, "checkpoint truncated");
+  const size_t N = g->N, R = g->nrep;
+  const char *o = (const char *)buf + sizeof(h);
+  HIPCHK(hipStreamSynchronize(g->stream));
+  HIPCHK(hipMemcpy(g->d_mol, o, sizeof(double) * NFIELD * N * R, hipMemcpyHostToDevice));
+  o += sizeof(double) * NFIELD * N * R;
+  std::memcpy(g->h_sc, o, sizeof(double) * NSC * R);
+  HIPCHK(hipMemcpy(g->d_sc, g->h_sc, sizeof(double) * NSC * R, hipMemcpyHostToDevice));
+  o += sizeof(double) * NSC * R;
+  HIPCHK(hipMemcpy(g->d_mid, o, sizeof(int) * N * R, hipMemcpyHostToDevice));
+  return 0;
+}
+
 // body_to_space (src/rotation.h:29-48)
 static void b2s(const double *a, const double *v, double *t) {
   const double a0sq = a[0] * a[0], a1sq = a[1] * a[1], a2sq = a[2] * a[2], a3sq = a[3] * a[3];
@@ -1705,6 +1780,37 @@ extern "C" int remd_rng_next(remd_rng *s) {
 }
 
 // ------------------------------------------------------- host exchange
+// Exchange pair list `type` (REMDInfo::ExchangeList, src/remdinfo.cxx:54-108):
+// condition slots (a, b) in list order; 0: T-even, 1: P-even, 2: T-odd,
+// 3: P-odd neighbours of the ntemp x npress grid (slot = t + p * ntemp).
+template <class F>
+static void for_each_pair(int dt, int dp, int type, F &&visit) {
+  if (type == 0) {
+    for (int j = 0; j < dp; ++j)
+      for (int i = 0; i < dt / 2; ++i) visit(2 * i + j * dt, 2 * i + 1 + j * dt);
+  } else if (type == 1) {
+    for (int j = 0; j < dp / 2; ++j)
+      for (int i = 0; i < dt; ++i) visit(i + 2 * j * dt, i + (2 * j + 1) * dt);
+  } else if (type == 2) {
+    for (int j = 0; j < dp; ++j)
+      for (int i = 0; i < dt / 2 - (dt + 1) % 2; ++i) visit(2 * i + j * dt + 1, 2 * i + 1 + j * dt + 1);
+  } else {
+    for (int j = 0; j < dp / 2 - (dp + 1) % 2; ++j)
+      for (int i = 0; i < dt; ++i) visit(i + (2 * j + 1) * dt, i + (2 * j + 2) * dt);
+  }
+}
+
+extern "C" int remd_exchange_pairs(int dt, int dp, int type, int *a, int *b) {
+  if (dt < 1 || dp < 1 || type < 0 || type > 3) return fail(REMD_EINVAL, "bad argument");
+  int k = 0;
+  for_each_pair(dt, dp, type, [&](int x, int y) {
+    if (a) a[k] = x;
+    if (b) b[k] = y;
+    k++;
+  });
+  return k;
+}
+
 extern "C" int remd_exchange_host(int dt, int dp, long step, int pconstant, const double *pot, const double *vol,
                                   int *index, double *T, double *P, unsigned char *isx, remd_rng *rng,
                                   int *accepted, int *npairs) {
@@ -1734,20 +1840,7 @@ extern "C" int remd_exchange_host(int dt, int dp, long step, int pconstant, cons
       isx[ind1] = isx[ind2] = 0;
     }
   };
-  const int ec = (int)(step % 4);
-  if (ec == 0) {
-    for (int j = 0; j < dp; ++j)
-      for (int i = 0; i < dt / 2; ++i) visit(2 * i + j * dt, 2 * i + 1 + j * dt);
-  } else if (ec == 1) {
-    for (int j = 0; j < dp / 2; ++j)
-      for (int i = 0; i < dt; ++i) visit(i + 2 * j * dt, i + (2 * j + 1) * dt);
-  } else if (ec == 2) {
-    for (int j = 0; j < dp; ++j)
-      for (int i = 0; i < dt / 2 - (dt + 1) % 2; ++i) visit(2 * i + j * dt + 1, 2 * i + 1 + j * dt + 1);
-  } else {
-    for (int j = 0; j < dp / 2 - (dp + 1) % 2; ++j)
-      for (int i = 0; i < dt; ++i) visit(i + (2 * j + 1) * dt, i + (2 * j + 2) * dt);
-  }
+  for_each_pair(dt, dp, (int)(step % 4), visit);
   if (npairs) *npairs = k;
   return nacc;
 }

@@ -188,3 +188,17 @@ def test_driver_rejects_unsupported_input(tmp_path):
         (tmp_path / "x.inp").write_text(text)
         res = subprocess.run([drv, str(tmp_path / "x.inp")], capture_output=True, text=True, timeout=60)
         assert res.returncode != 0 and word in res.stderr, res.stderr
+
+
+@pytest.mark.parametrize("nt,npr", [(4, 1), (16, 4), (5, 3), (2, 2), (3, 1)])
+def test_exchange_pairs_match_reference_lists(golden, nt, npr):
+    """remd_exchange_pairs gives the reference's four exchange pair lists
+    (REMDInfo::ExchangeList, src/remdinfo.cxx:54-108; fixture from the
+    reference build) in list order."""
+    d = golden("ref_remdinfo.npz")
+    for typ in range(4):
+        a, b = ra.exchange_pairs(nt, npr, typ)
+        n = int(d[f"len_{nt}x{npr}"][typ])
+        assert a.shape[0] == n
+        ref = d[f"lists_{nt}x{npr}"][typ, :n]
+        assert np.array_equal(np.stack([a, b], 1), ref)
