@@ -206,6 +206,15 @@ int remd_gpu_set_conditions(remd_gpu *g, const double *T, const double *P);
  * intervals after the batch changes time the candidate plans; results do
  * not depend on the plan. */
 int remd_batch_execute(remd_batch *b);
+/* remd_batch_execute in two halves, so that several batches (each on its
+ * own stream) run concurrently: launch enqueues the interval and the result
+ * copies and returns; poll returns 1 once they are done (0 before); wait
+ * blocks until then and finishes the bookkeeping (abort check, timing,
+ * autotuner).  Between launch and wait the batch's jobs must not be staged,
+ * read or changed, and the batch must not be changed (REMD_ESTATE). */
+int remd_batch_launch(remd_batch *b);
+int remd_batch_poll(remd_batch *b);
+int remd_batch_wait(remd_batch *b);
 /* Time candidate packing plans on the first intervals (default on); off:
  * keep the cost model's best plan. */
 int remd_batch_set_autotune(remd_batch *b, int on);

@@ -103,6 +103,9 @@ def lib():
     L.remd_batch_add.argtypes = [C.c_void_p, C.c_void_p]
     L.remd_batch_remove.argtypes = [C.c_void_p, C.c_void_p]
     L.remd_batch_execute.argtypes = [C.c_void_p]
+    L.remd_batch_launch.argtypes = [C.c_void_p]
+    L.remd_batch_poll.argtypes = [C.c_void_p]
+    L.remd_batch_wait.argtypes = [C.c_void_p]
     L.remd_batch_set_autotune.argtypes = [C.c_void_p, C.c_int]
     L.remd_batch_set_parts.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.c_int]
     L.remd_batch_last_kernel_ms.argtypes = [C.c_void_p, _dp]
@@ -363,6 +366,21 @@ class Batch:
     def execute(self):
         _check(lib().remd_batch_execute(self.h))
 
+    def launch(self):
+        """remd_batch_launch: enqueue one interval and return."""
+        _check(lib().remd_batch_launch(self.h))
+
+    def poll(self) -> bool:
+        """remd_batch_poll: True once the launched interval is done."""
+        rc = lib().remd_batch_poll(self.h)
+        if rc < 0:
+            _check(rc)
+        return rc == 1
+
+    def wait(self):
+        """remd_batch_wait: block until the launched interval is done."""
+        _check(lib().remd_batch_wait(self.h))
+
     def set_autotune(self, on: bool):
         """Time candidate packing plans on the first intervals after the
         batch changes (default), or keep the cost model's best plan."""
@@ -410,6 +428,12 @@ class GlibcRand:
 
     def uniform(self) -> float:
         return self.rand() / 2147483647.0
+
+    def getstate(self) -> bytes:
+        return bytes(self.s)
+
+    def setstate(self, state: bytes):
+        C.memmove(C.byref(self.s), state, C.sizeof(self.s))
 
 
 def exchange_host(ntemp, npress, step, pconstant, pot, vol, index, T, P, rng: "GlibcRand"):

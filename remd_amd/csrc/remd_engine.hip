@@ -189,6 +189,8 @@ struct remd_batch {
   hipStream_t cstream[NCLASS] = {nullptr};
   hipEvent_t cdone[NCLASS] = {nullptr};
   hipEvent_t ev0 = nullptr, ev1 = nullptr, efork = nullptr;
+  hipEvent_t edone = nullptr;  // end of a launch's result copies (remd_batch_poll)
+  bool in_flight = false;      // launched, not yet waited for
   double last_ms = 0.0;
 };
 
@@ -1332,6 +1334,7 @@ extern "C" int remd_batch_create(remd_batch **out) {
   if (e == hipSuccess) e = hipEventCreate(&b->ev0);
   if (e == hipSuccess) e = hipEventCreate(&b->ev1);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&b->efork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->edone, hipEventDisableTiming);
   for (int c = 0; c < NCLASS && e == hipSuccess; c++) {
     e = hipStreamCreateWithFlags(&b->cstream[c], hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->cdone[c], hipEventDisableTiming);
@@ -1348,6 +1351,7 @@ extern "C" int remd_batch_create(remd_batch **out) {
 
 extern "C" int remd_batch_add(remd_batch *b, remd_gpu *g) {
   if (!b || !g) return fail(REMD_EINVAL, "null argument");
+  if (b->in_flight) return fail(REMD_ESTATE, "the batch has an interval in flight (remd_batch_wait first)");
   if (std::find(b->jobs.begin(), b->jobs.end(), g) != b->jobs.end()) return 0;
   b->jobs.push_back(g);
   b->dirty = true;
@@ -1356,6 +1360,7 @@ extern "C" int remd_batch_add(remd_batch *b, remd_gpu *g) {
 
 extern "C" int remd_batch_remove(remd_batch *b, remd_gpu *g) {
   if (!b || !g) return fail(REMD_EINVAL, "null argument");
+  if (b->in_flight) return fail(REMD_ESTATE, "the batch has an interval in flight (remd_batch_wait first)");
   auto it = std::find(b->jobs.begin(), b->jobs.end(), g);
   if (it == b->jobs.end()) return fail(REMD_EINVAL, "job not in batch");
   b->jobs.erase(it);
@@ -1372,6 +1377,7 @@ extern "C" int remd_batch_set_autotune(remd_batch *b, int on) {
 
 extern "C" int remd_batch_set_parts(remd_batch *b, const int *parts, int n) {
   if (!b) return fail(REMD_EINVAL, "null batch");
+  if (b->in_flight) return fail(REMD_ESTATE, "the batch has an interval in flight (remd_batch_wait first)");
   if (!parts) {
     b->forced.clear();
   } else {
@@ -1633,8 +1639,9 @@ static void add_neighbours(remd_batch *b) {
   }
 }
 
-extern "C" int remd_batch_execute(remd_batch *b) {
+extern "C" int remd_batch_launch(remd_batch *b) {
   if (!b) return fail(REMD_EINVAL, "null batch");
+  if (b->in_flight) return fail(REMD_ESTATE, "the batch's previous interval is still in flight");
   if (b->jobs.empty()) return 0;
   for (remd_gpu *g : b->jobs)
     if (!g->resident && g->nt != 256 && g->nt != 512)
@@ -1648,6 +1655,14 @@ extern "C" int remd_batch_execute(remd_batch *b) {
   if (b->split) HIPCHK(hipMemsetAsync(b->d_sync, 0, sizeof(unsigned long long) * (b->res_n + 1), b->stream));
   for (size_t k = 0; k < b->jobs.size(); k++) b->jobs[k]->last_parts = b->jobs[k]->resident ? b->plan[k] : 1;
   HIPCHK(hipEventRecord(b->ev0, b->stream));
+  // the global-state classes start with the resident launch when no replica
+  // of it is split over workgroups; otherwise after it, so that no
+  // global-state workgroup holds a CU a waiting part needs
+  bool fork = false;
+  if (!b->split) {
+    HIPCHK(hipEventRecord(b->efork, b->stream));
+    fork = true;
+  }
   if (b->nblk) {
     LaunchArgs a;
     a.reps = b->d_table;
@@ -1664,8 +1679,6 @@ extern "C" int remd_batch_execute(remd_batch *b) {
     a.abort = (unsigned *)(b->d_sync + b->res_n);
     HIPCHK(launch_resident_batch(a, b->res_lds, b->stream));
   }
-  // then the global-state classes, concurrently
-  bool fork = false;
   for (int c = 0; c < NCLASS; c++) {
     if (!b->cls_n[c]) continue;
     if (b->cls_lds[c] > device_lds_limit()) return fail(REMD_EINVAL, "batch LDS exceeds the device limit");
@@ -1699,6 +1712,24 @@ extern "C" int remd_batch_execute(remd_batch *b) {
     HIPCHK(hipMemcpyAsync(b->h_abort, b->d_sync + b->res_n, sizeof(unsigned), hipMemcpyDeviceToHost, b->stream));
   for (remd_gpu *g : b->jobs)
     HIPCHK(hipMemcpyAsync(g->h_sc, g->d_sc, sizeof(double) * NSC * g->nrep, hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipEventRecord(b->edone, b->stream));
+  b->in_flight = true;
+  return 0;
+}
+
+extern "C" int remd_batch_poll(remd_batch *b) {
+  if (!b) return fail(REMD_EINVAL, "null batch");
+  if (!b->in_flight) return 1;
+  const hipError_t e = hipEventQuery(b->edone);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  return fail(REMD_EHIP, hipGetErrorString(e));
+}
+
+extern "C" int remd_batch_wait(remd_batch *b) {
+  if (!b) return fail(REMD_EINVAL, "null batch");
+  if (!b->in_flight) return 0;
+  b->in_flight = false;
   HIPCHK(hipStreamSynchronize(b->stream));
   if (b->split && *b->h_abort != 0u) {
     *b->h_abort = 0u;
@@ -1726,6 +1757,12 @@ extern "C" int remd_batch_execute(remd_batch *b) {
   return 0;
 }
 
+extern "C" int remd_batch_execute(remd_batch *b) {
+  int rc = remd_batch_launch(b);
+  if (rc) return rc;
+  return remd_batch_wait(b);
+}
+
 extern "C" int remd_batch_last_kernel_ms(remd_batch *b, double *ms) {
   if (!b || !ms) return fail(REMD_EINVAL, "null argument");
   *ms = b->last_ms;
@@ -1746,6 +1783,7 @@ extern "C" void remd_batch_destroy(remd_batch *b) {
   if (b->ev0) hipEventDestroy(b->ev0);
   if (b->ev1) hipEventDestroy(b->ev1);
   if (b->efork) hipEventDestroy(b->efork);
+  if (b->edone) hipEventDestroy(b->edone);
   if (b->stream) hipStreamDestroy(b->stream);
   delete b;
 }

@@ -162,7 +162,8 @@ def uneven_batch_study(intervals=10, warmup=16):
     return out
 
 
-def served_study(njobs=24, rate=20.0, intervals=50, seed=5, nmol_hi=2048):
+def served_study(njobs=24, rate=20.0, intervals=50, seed=5, nmol_hi=2048, lane_bounds=None, split=True,
+                 reqs=None):
     """The Poisson stream of REMD jobs (mean `rate` jobs/s, nmol log-uniform
     64-nmol_hi, TIP3P/4P/5P, bulk/tube/slit, 8-128 replicas, `intervals`
     exchange intervals each) served with continuous batching under a 200 GB
@@ -173,12 +174,14 @@ def served_study(njobs=24, rate=20.0, intervals=50, seed=5, nmol_hi=2048):
 
     from .serve import GPUBackend, Request, Server, latency_percentiles, poisson_stream
 
-    reqs = poisson_stream(seed=seed, njobs=njobs, rate=rate, intervals=intervals, nmol_hi=nmol_hi)
+    if reqs is None:
+        reqs = poisson_stream(seed=seed, njobs=njobs, rate=rate, intervals=intervals, nmol_hi=nmol_hi)
     steps = sum(r.intervals * r.system.interval * r.system.nreplica for r in reqs)
-    _progress(f"{njobs} jobs generated and relaxed")
-    be = GPUBackend()
+    _progress(f"{len(reqs)} jobs generated and relaxed")
+    be = GPUBackend(split=split)
+    kw = {} if lane_bounds is None else {"lane_bounds": lane_bounds}
     t0 = time.perf_counter()
-    res = Server(be, hbm_budget=200e9).run(reqs)
+    res = Server(be, hbm_budget=200e9, **kw).run(reqs)
     wall = time.perf_counter() - t0
     _progress(f"served stream: {wall:.3f} s")
     be.close()
@@ -194,7 +197,7 @@ def served_study(njobs=24, rate=20.0, intervals=50, seed=5, nmol_hi=2048):
     wall1 = time.perf_counter() - t0
     _progress(f"one job at a time: {wall1:.3f} s")
     turn = [res[r.job_id].finished - r.arrival for r in reqs]
-    return dict(jobs=njobs, replica_steps=steps, served=steps / wall, one_job_at_a_time=steps / wall1,
+    return dict(jobs=len(reqs), replica_steps=steps, served=steps / wall, one_job_at_a_time=steps / wall1,
                 interval_latency_ms={"p50": p50 * 1e3, "p99": p99 * 1e3},
                 one_job_at_a_time_latency_ms={"p50": float(np.percentile(lat, 50)) * 1e3,
                                               "p99": float(np.percentile(lat, 99)) * 1e3},
