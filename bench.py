@@ -164,6 +164,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-studies", action="store_true",
+                    help="skip the uneven-batch, served-stream and precision figures of the headline line")
     ap.add_argument("--plain-steps", type=int, default=6)
     ap.add_argument("--workload", default="tube64", choices=["tube64", "uneven", "served", "precision"],
                     help="tube64: the headline line; the others print one line of their own study")
@@ -277,7 +279,9 @@ def main():
         "data": "synthetic (seeded placement, device-equilibrated)",
         "config": {
             "workload": "nanotube TIP4P, 144 mol/replica, R 7 A, Lz 50 A, 16T x 4P = 64 replicas, "
-                        "rcut 9 A, dt 2 fs, 100 steps/interval, NP thermostat + 1D barostat",
+                        "rcut 9 A, dt 2 fs, 100 steps/interval, NP thermostat + 1D barostat; COMs placed "
+                        "at r < 4.2 A (R - 2.8: the wall energy is about +500 kJ/mol at R - 2 A), min O-O "
+                        "2.6 A, relaxed 300 x 0.5 fs on the device",
             "replicas": nrep, "nmol": system.nmol, "steps_per_interval": interval,
             "launch": {"path": "resident" if path == ra.PATH_RESIDENT else "global",
                        "workgroups_per_replica": parts, "block_threads": shape["block_threads"],
@@ -308,7 +312,29 @@ def main():
     job.close()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(system, system.T, system.P)
+    if not args.no_studies:
+        out["studies"] = studies()
     print(json.dumps(out))
+
+
+def studies():
+    """The other benchmark configurations, each at its own size, after the
+    timed headline loop: the uneven batch (8 tubes, 256 replicas), the served
+    Poisson stream (24 jobs, nmol up to 2048, 50 intervals) and the
+    precision study (512 TIP5P slit, 128 replicas, 10 000-step drift)."""
+    from remd_amd import workloads
+
+    out = {}
+    for name, fn in (("uneven_batch", workloads.uneven_batch_study), ("served_stream", workloads.served_study),
+                     ("precision", workloads.precision_study)):
+        t0 = time.perf_counter()
+        try:
+            out[name] = fn()
+        except Exception as e:  # one failed study must not lose the headline figures
+            out[name] = {"error": f"{type(e).__name__}: {e}"}
+        out[name]["wall_s"] = round(time.perf_counter() - t0, 1)
+        print(f"study {name}: {out[name]['wall_s']} s", file=sys.stderr, flush=True)
+    return out
 
 
 if __name__ == "__main__":
