@@ -752,8 +752,7 @@ struct PairK {
   PT rs2;  // screening radius^2, slightly above rc^2 (pair_acc decides)
   PT sig2, eps4, eps48;
   PT qq[NS][NS];
-  PT L[3], Li[3];
-  int per[3];
+  PT L[3], Li[3];  // L: the box length on periodic axes, 0 on the others (no wrap there)
 };
 
 template <int NS, typename AT = double>
@@ -785,7 +784,7 @@ __device__ __forceinline__ void pair_acc(const PT *gi, const PT (&ai)[NS][3], co
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     dg[d] = gi[d] - rj[d];
-    if (k.per[d]) dg[d] -= k.L[d] * rnd(dg[d] * k.Li[d]);
+    dg[d] -= k.L[d] * rnd(dg[d] * k.Li[d]);
   }
   dg[2] = self ? PT(1.0e4) : dg[2];
   const PT r2 = dg[0] * dg[0] + dg[1] * dg[1] + dg[2] * dg[2];
@@ -838,24 +837,28 @@ __device__ __forceinline__ void pair_acc(const PT *gi, const PT (&ai)[NS][3], co
       fs[a][2] += f * z;
     }
   }
+  // switched site forces into the item's sums (one multiply-add each) and
+  // the pair's COM force -dS/dr E g + S sum_a f_a for the virial
   const PT cr = -dswr * (elj + ecl);
-  PT F0 = cr * dg[0], F1 = cr * dg[1], F2 = cr * dg[2];
-  acc.corr[0] += (AT)F0;
-  acc.corr[1] += (AT)F1;
-  acc.corr[2] += (AT)F2;
+  PT fsum[3] = {fs[0][0], fs[0][1], fs[0][2]};
+#pragma unroll
+  for (int a = 1; a < NS; a++) {
+    fsum[0] += fs[a][0];
+    fsum[1] += fs[a][1];
+    fsum[2] += fs[a][2];
+  }
 #pragma unroll
   for (int a = 0; a < NS; a++) {
-    const PT sx = sw * fs[a][0], sy = sw * fs[a][1], sz = sw * fs[a][2];
-    acc.site[a][0] += (AT)sx;
-    acc.site[a][1] += (AT)sy;
-    acc.site[a][2] += (AT)sz;
-    F0 += sx;
-    F1 += sy;
-    F2 += sz;
+    acc.site[a][0] += (AT)(sw * fs[a][0]);
+    acc.site[a][1] += (AT)(sw * fs[a][1]);
+    acc.site[a][2] += (AT)(sw * fs[a][2]);
   }
-  acc.vir[0] += (AT)(F0 * dg[0]);
-  acc.vir[1] += (AT)(F1 * dg[1]);
-  acc.vir[2] += (AT)(F2 * dg[2]);
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    acc.corr[d] += (AT)(cr * dg[d]);
+    const PT F = cr * dg[d] + sw * fsum[d];
+    acc.vir[d] += (AT)(F * dg[d]);
+  }
   acc.lj += (AT)(sw * elj);
   acc.clmb += (AT)(sw * ecl);
 }
@@ -915,7 +918,7 @@ __device__ __forceinline__ void pair_window_run(const PT *rec, int N, int i, int
 #pragma unroll
         for (int d = 0; d < 3; d++) {
           dg[d] = gi[d] - c[d];
-          if (k.per[d]) dg[d] -= k.L[d] * rnd(dg[d] * k.Li[d]);
+          dg[d] -= k.L[d] * rnd(dg[d] * k.Li[d]);
         }
         const PT r2 = dg[0] * dg[0] + dg[1] * dg[1] + dg[2] * dg[2];
         if (j != i && r2 < k.rs2) mask |= 1u << u;
@@ -1055,9 +1058,8 @@ __device__ void force_pass(JRef J, double *mol, int N, Sc &c, const Lds &S,
   k.rs2 = k.rc2 * (PT)1.0001;
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    k.L[d] = (PT)c.L[d];
+    k.L[d] = J.per[d] ? (PT)c.L[d] : PT(0);
     k.Li[d] = (PT)(1.0 / c.L[d]);
-    k.per[d] = J.per[d];
   }
 
   const int C = J.chunks;
@@ -1191,8 +1193,23 @@ __device__ void force_pass(JRef J, double *mol, int N, Sc &c, const Lds &S,
 // ----------------------------------------------------------- one step
 // First half of ExecuteStep (src/integrator.cxx:651-678): D6, D5 scalars,
 // D4, then per molecule D5 (old box), D3 D2 D1 D2 D3 (new box), PsPv, D4.
-template <int NT>
+// First half of a step (src/integrator.cxx:646-685).  With FUSE the second
+// half of the previous step (src/integrator.cxx:686-703) runs first, in the
+// same molecule loop: its scalar part (D5 scalars, D6) before the loop and
+// its D5 on each molecule, with the box and s it saw, right before this
+// step's D5 on that molecule -- the same operations in the same order as
+// second_half() followed by first_half(), one pass over the molecules.
+template <int NT, bool FUSE = false>
 __device__ void first_half(JRef J, double *mol, int N, Sc &c, double *red) {
+  Sc c_prev;  // D5 of the previous step's second half: box and s before its scalars
+  if (FUSE) {
+    c_prev.s = c.s;
+    c_prev.L[0] = c.L[0];
+    c_prev.L[1] = c.L[1];
+    c_prev.L[2] = c.L[2];
+    D5_scalars(J, c);
+    D6(J, c);
+  }
   D6(J, c);
   Sc c_old;  // D5 on the molecules uses the box and s from before D4
   c_old.s = c.s;
@@ -1205,23 +1222,24 @@ __device__ void first_half(JRef J, double *mol, int N, Sc &c, double *red) {
   const bool needPv = (J.pmode > 0) || T1;
   double sums[4] = {0, 0, 0, 0};  // sum_Pv[3], sum_Ps
   // PsPv factors (box and s are fixed through the molecule loop)
-  double ipv[3];
+  // reciprocals of s and of the box once; the factors below are products
+  const double si = 1.0 / c.s;
+  double Li[3], ipv[3];
 #pragma unroll
-  for (int d = 0; d < 3; d++) ipv[d] = 1.0 / (c.L[d] * c.L[d] * c.s * c.s);
+  for (int d = 0; d < 3; d++) {
+    Li[d] = 1.0 / c.L[d];
+    ipv[d] = (Li[d] * Li[d]) * (si * si);
+  }
   const double logs = T1 ? log(c.s) : 0.0;
   double fpv = 0.0;
   if (J.pmode == 1) fpv = c.s / (volume(J, c.L[0], c.L[1], c.L[2]) * 3.0) * J.dthalf;
-  if (J.pmode == 2) fpv = c.s / c.L[2] * J.dthalf;
+  if (J.pmode == 2) fpv = c.s * Li[2] * J.dthalf;
   if (J.pmode == 3) fpv = c.s / (2.0 * c.L[0] * c.L[1]) * J.dthalf;
   {
-    const double coef_xi = 0.25 / c.s;
-    const double si = 1.0 / c.s;
+    const double coef_xi = 0.25 * si;
     double coef_r[3];
 #pragma unroll
-    for (int d = 0; d < 3; d++) {
-      const double Li = 1.0 / c.L[d];
-      coef_r[d] = Li * Li * si * J.dt;
-    }
+    for (int d = 0; d < 3; d++) coef_r[d] = Li[d] * Li[d] * si * J.dt;
     const double *I = (const double *)J.inertia;
     const double *iI = (const double *)J.inv_inertia;
     for (int i = threadIdx.x; i < N; i += NT) {
@@ -1231,6 +1249,7 @@ __device__ void first_half(JRef J, double *mol, int N, Sc &c, double *red) {
       double q[4] = {mol[F_Q0 * N + i], mol[F_Q1 * N + i], mol[F_Q2 * N + i], mol[F_Q3 * N + i]};
       double p[4] = {mol[F_P0 * N + i], mol[F_P1 * N + i], mol[F_P2 * N + i], mol[F_P3 * N + i]};
       const double n[4] = {mol[F_N0 * N + i], mol[F_N1 * N + i], mol[F_N2 * N + i], mol[F_N3 * N + i]};
+      if (FUSE) D5_mol(J, c_prev, v, f, p, q, n);
       D5_mol(J, c_old, v, f, p, q, n);
       double xi;
       xi = rotate(q, p, 2, coef_xi, iI[2], J.dthalf);
@@ -1707,9 +1726,8 @@ __device__ void force_range(JRef J, double *mol, int N, const Sc &c, const LdsR 
   k.rs2 = k.rc2 * (PT)1.0001;
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    k.L[d] = (PT)c.L[d];
+    k.L[d] = J.per[d] ? (PT)c.L[d] : PT(0);
     k.Li[d] = (PT)(1.0 / c.L[d]);
-    k.per[d] = J.per[d];
   }
   const int C = J.chunks;
   const int B = (NT / C) / EB * EB;  // molecules per round, whole energy blocks
@@ -2049,17 +2067,21 @@ __device__ __attribute__((always_inline)) void run_resident(const LaunchArgs &a,
     if (J.tmode == 2) velocity_scaling<NT>(J, mol, N, c, S.red);
     double *pt = lead ? S.ptimer : nullptr;
     (void)pt;
+    // the second half of step st - 1 runs fused into the first half of step
+    // st (not with per-step velocity scaling, which comes between them)
+    const bool fuse = J.tmode != 2;
     for (int st = 0; st < J.interval; ++st) {
       PT_DECL
       if (J.tmode == 2) velocity_scaling<NT>(J, mol, N, c, S.red);
-      first_half<NT>(J, mol, N, c, S.red);
+      if (fuse && st > 0) first_half<NT, true>(J, mol, N, c, S.red);
+      else first_half<NT>(J, mol, N, c, S.red);
       PT_MARK(0, pt);
       if ((st % J.sort_interval) == 0)
         rank_sort_resident<NT, NS>(J, mol, mid, N, J.sort_axis == 0 ? c.L[0] : c.L[2], J.margin, S, win_dbg);
       PT_MARK(1, pt);
       if (!forces_resident<NT, NS, PT>(J, mol, N, c, S, g, i0, i1, site_dbg, pt)) return;
       PT_MARK(2, pt);
-      second_half<NT>(J, mol, N, c);
+      if (!fuse || st == J.interval - 1) second_half<NT>(J, mol, N, c);
       PT_MARK(3, pt);
     }
     calc_properties<NT>(J, mol, N, c, sc_out, S.red);
