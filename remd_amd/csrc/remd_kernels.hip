@@ -52,6 +52,7 @@ __device__ __forceinline__ T *gen(REMD_AS_GLOBAL T *p) {
 typedef const REMD_AS_CONST JobParams &JRef;
 
 
+
 // Optional phase timers (constant 100 MHz clock), accumulated by thread 0
 // into 16 accumulators (the S_DIAG0.. words of the replica's scalar block,
 // or LDS on the resident path); compiled out by default.
@@ -1837,70 +1838,161 @@ __device__ void force_range(JRef J, double *mol, int N, const Sc &c, const LdsR 
     }
     __syncthreads();
     PT_MARK(12, pt_sc);
-    const bool mine = t < ie - ib;
-    double v[NPART];
+    // Chunk partials -> per-molecule values, one (value group, molecule)
+    // per thread: group 0 the COM force (partials 0-2), 1 the quaternion
+    // torque (3-6), 2 the virial (7-9), 3 lj, clmb, wall (10-12).  Each
+    // value is its chunks added in chunk order.  A group spans M4 lanes
+    // (whole lane quads), so the EB = 4 molecules of an energy block sit on
+    // one lane quad (rounds start on block boundaries).
+    const int M = ie - ib;
+    const int M4 = (M + EB - 1) / EB * EB;
+    for (int w = t; w < 4 * M4; w += NT) {
+      const int grp = w / M4;
+      const int li = w - grp * M4;
+      const bool mine = li < M;
+      const int u0 = grp == 0 ? 0 : (grp == 1 ? 3 : (grp == 2 ? 7 : 10));
+      const int nu = grp == 1 ? 4 : 3;
+      double v[4] = {0.0, 0.0, 0.0, 0.0};
+      if (mine) {
+        const double *P = S.part + (size_t)u0 * NT + li * C;
+        // chunk-outer so each chunk's loads issue together
 #pragma unroll
-    for (int u = 0; u < NPART; u++) v[u] = 0.0;
-    if (mine) {
-      const int i = ib + t;
-      const double *P = S.part + t * C;
-      // chunks in order for every value; chunk-outer so each chunk's loads
-      // issue together
+        for (int u = 0; u < 4; u++)
+          if (u < nu) v[u] = P[u * NT];
+        for (int cc = 1; cc < C; cc++) {
+          double x[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int u = 0; u < NPART; u++) v[u] = P[u * NT];
-      for (int cc = 1; cc < C; cc++) {
-        double w[NPART];
+          for (int u = 0; u < 4; u++)
+            if (u < nu) x[u] = P[u * NT + cc];
 #pragma unroll
-        for (int u = 0; u < NPART; u++) w[u] = P[u * NT + cc];
-#pragma unroll
-        for (int u = 0; u < NPART; u++) v[u] += w[u];
+          for (int u = 0; u < 4; u++)
+            if (u < nu) v[u] += x[u];
+        }
+        // F, n into the state (and the exchange row)
+        const int i = ib + li;
+        if (grp == 0) {
+          mol[F_FX * N + i] = v[0];
+          mol[F_FY * N + i] = v[1];
+          mol[F_FZ * N + i] = v[2];
+          if (xrow) {
+            double *o = xrow + (size_t)i * XR;
+            o[0] = v[0];
+            o[1] = v[1];
+            o[2] = v[2];
+          }
+        } else if (grp == 1) {
+          mol[F_N0 * N + i] = v[0];
+          mol[F_N1 * N + i] = v[1];
+          mol[F_N2 * N + i] = v[2];
+          mol[F_N3 * N + i] = v[3];
+          if (xrow) {
+            double *o = xrow + (size_t)i * XR;
+            o[3] = v[0];
+            o[4] = v[1];
+            o[5] = v[2];
+            o[6] = v[3];
+          }
+        }
       }
-      // F, n into the state (and the exchange row)
-      if (xrow) {
-        double *o = xrow + (size_t)i * XR;
+      // energy-block sums (groups 2 and 3): lane 4k adds its quad's values
+      // in molecule order (absent molecules are zeros); quads are whole, so
+      // every lane a quad_perm reads is active
+      double bs[3];
 #pragma unroll
-        for (int u = 0; u < 7; u++) o[u] = v[u];
+      for (int u = 0; u < 3; u++) {
+        const double x1 = dpp_mov<0x55>(v[u]);  // quad_perm [1, 1, 1, 1]
+        const double x2 = dpp_mov<0xAA>(v[u]);  // quad_perm [2, 2, 2, 2]
+        const double x3 = dpp_mov<0xFF>(v[u]);  // quad_perm [3, 3, 3, 3]
+        bs[u] = ((v[u] + x1) + x2) + x3;
       }
-      mol[F_FX * N + i] = v[0];
-      mol[F_FY * N + i] = v[1];
-      mol[F_FZ * N + i] = v[2];
-      mol[F_N0 * N + i] = v[3];
-      mol[F_N1 * N + i] = v[4];
-      mol[F_N2 * N + i] = v[5];
-      mol[F_N3 * N + i] = v[6];
-      if (site_dbg) {
-        for (int u = NPART; u < NV; u++) {
-          double sum = P[u * NT];
-          for (int cc = 1; cc < C; cc++) sum += P[u * NT + cc];
-          site_dbg[i * NS * 3 + (u - NPART)] = sum;
+      if (grp >= 2 && mine && (li & 3) == 0) {
+        const int b = (ib + li) / EB;
+        const int r0 = grp == 3 ? 0 : 3;  // block rows: lj, clmb, wall, vir0, vir1, vir2
+#pragma unroll
+        for (int u = 0; u < 3; u++) Bk[(r0 + u) * nblk + b] = bs[u];
+        if (xblk) {
+#pragma unroll
+          for (int u = 0; u < 3; u++) xblk[(size_t)b * XB + r0 + u] = bs[u];
         }
       }
     }
-    // energy-block sums: the EB = 4 molecules of a block sit on one lane
-    // quad (rounds start on block boundaries); lane 4k adds them in
-    // molecule order (absent molecules are zeros)
-    {
-      const double ev[6] = {v[10], v[11], v[12], v[7], v[8], v[9]};
-      double bs[6];
-#pragma unroll
-      for (int u = 0; u < 6; u++) {
-        const double x1 = dpp_mov<0x55>(ev[u]);  // quad_perm [1, 1, 1, 1]
-        const double x2 = dpp_mov<0xAA>(ev[u]);  // quad_perm [2, 2, 2, 2]
-        const double x3 = dpp_mov<0xFF>(ev[u]);  // quad_perm [3, 3, 3, 3]
-        bs[u] = ((ev[u] + x1) + x2) + x3;
-      }
-      if (mine && (t & 3) == 0) {
-        const int b = (ib + t) / EB;
-#pragma unroll
-        for (int u = 0; u < 6; u++) Bk[u * nblk + b] = bs[u];
-        if (xblk) {
-#pragma unroll
-          for (int u = 0; u < 6; u++) xblk[(size_t)b * XB + u] = bs[u];
+    if (site_dbg) {
+      for (int li = t; li < M; li += NT) {
+        const double *P = S.part + li * C;
+        for (int u = NPART; u < NV; u++) {
+          double sum = P[u * NT];
+          for (int cc = 1; cc < C; cc++) sum += P[u * NT + cc];
+          site_dbg[(ib + li) * NS * 3 + (u - NPART)] = sum;
         }
       }
     }
     __syncthreads();
     PT_MARK(13, pt_sc);
+  }
+}
+
+// Exchange row of molecule k (COM force, quaternion torque) written by
+// another part.  `direct`: device-scope loads that bypass the CU's L1 (the
+// same-XCD barrier drains stores only to L2).
+__device__ inline void xrow_load(const double *o, bool direct, double (&v)[7]) {
+  if (direct) {
+#pragma unroll
+    for (int u = 0; u < 7; u++) v[u] = __hip_atomic_load(o + u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+#pragma unroll
+    for (int u = 0; u < 7; u++) v[u] = o[u];
+  }
+}
+
+__device__ inline void xrow_store(double *mol, int N, int k, const double (&v)[7]) {
+  mol[F_FX * N + k] = v[0];
+  mol[F_FY * N + k] = v[1];
+  mol[F_FZ * N + k] = v[2];
+  mol[F_N0 * N + k] = v[3];
+  mol[F_N1 * N + k] = v[4];
+  mol[F_N2 * N + k] = v[5];
+  mol[F_N3 * N + k] = v[6];
+}
+
+// Totals of the six block sums (lj, clmb, wall, virial diagonal) over the
+// replica's nblk energy blocks, computed by every wave for itself (no LDS
+// round trip, no workgroup barrier).  The order is a fixed tree of NT
+// slots: slot s adds blocks s, s + NT, ... in order, the slots of each
+// 64-slot group are added by wave_sum, and the groups are added in group
+// order after a 0.0 -- the order of block_sum<NT> over slots, whatever the
+// parts.  Block b comes from this part's LDS rows (Bk, blocks [b0, b1),
+// or all of them when xblk is null) or from the exchange area.
+template <int NT>
+__device__ inline void block_totals(const double *Bk, const double *xblk, int nblk, int b0, int b1,
+                                    bool direct, double (&tot)[6]) {
+  constexpr int NW = NT / WAVE;
+  const int lane = threadIdx.x & (WAVE - 1);
+#pragma unroll
+  for (int u = 0; u < 6; u++) tot[u] = 0.0;
+  for (int ww = 0; ww < NW; ww++) {
+    double p[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (ww * WAVE < nblk) {  // wave-uniform
+      for (int b = ww * WAVE + lane; b < nblk; b += NT) {
+        double e[6];
+        if (xblk == nullptr || (b >= b0 && b < b1)) {
+#pragma unroll
+          for (int u = 0; u < 6; u++) e[u] = Bk[u * nblk + b];
+        } else if (direct) {
+#pragma unroll
+          for (int u = 0; u < 6; u++)
+            e[u] = __hip_atomic_load(xblk + (size_t)b * XB + u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+#pragma unroll
+          for (int u = 0; u < 6; u++) e[u] = xblk[(size_t)b * XB + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 6; u++) p[u] += e[u];
+      }
+#pragma unroll
+      for (int u = 0; u < 6; u++) p[u] = wave_sum(p[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < 6; u++) tot[u] += p[u];
   }
 }
 
@@ -1925,55 +2017,31 @@ __device__ bool forces_resident(JRef J, double *mol, int N, Sc &c, const LdsR &S
   double *Bk = S.E;  // 6 x nblk block sums
   force_range<NT, NS, PT>(J, mol, N, c, S, i0, i1, xrow, Bk, nblk, xblk, site_dbg, pt_sc);
   PT_MARK(5, pt_sc);
-  if (g.parts > 1) {
+  const bool split = g.parts > 1;
+  if (split) {
     if (!group_wait<NT>(g, S.flag, pt_sc)) return false;
     PT_MARK(6, pt_sc);
-    const bool direct = g.same_xcd && !(g.mode & 2);
-    // the other parts' rows and block sums, all loads of a thread issued
-    // together
-    const int nrow = N > nblk ? N : nblk;
-    for (int k = threadIdx.x; k < nrow; k += NT) {
-      const bool row = k < N && !(k >= i0 && k < i1);
-      const bool blk = k < nblk && !(k >= b0 && k < b1);
-      const double *o = xrow + (size_t)(row ? k : 0) * XR;
-      const double *ob = xblk + (size_t)(blk ? k : 0) * XB;
-      double v[7], e[6];
-      if (direct) {
-#pragma unroll
-        for (int u = 0; u < 7; u++) v[u] = row ? __hip_atomic_load(o + u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
-#pragma unroll
-        for (int u = 0; u < 6; u++) e[u] = blk ? __hip_atomic_load(ob + u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
-      } else {
-#pragma unroll
-        for (int u = 0; u < 7; u++) v[u] = row ? o[u] : 0.0;
-#pragma unroll
-        for (int u = 0; u < 6; u++) e[u] = blk ? ob[u] : 0.0;
-      }
-      if (row) {
-        mol[F_FX * N + k] = v[0];
-        mol[F_FY * N + k] = v[1];
-        mol[F_FZ * N + k] = v[2];
-        mol[F_N0 * N + k] = v[3];
-        mol[F_N1 * N + k] = v[4];
-        mol[F_N2 * N + k] = v[5];
-        mol[F_N3 * N + k] = v[6];
-      }
-      if (blk) {
-#pragma unroll
-        for (int u = 0; u < 6; u++) Bk[u * nblk + k] = e[u];
-      }
+  }
+  const bool direct = split && g.same_xcd && !(g.mode & 2);
+  // the other parts' rows: the first row of each thread is loaded before
+  // the block totals' loads so the two latencies overlap, and written to
+  // the state after them
+  int k = threadIdx.x;
+  const bool row0 = split && k < N && !(k >= i0 && k < i1);
+  double rv[7];
+  if (row0) xrow_load(xrow + (size_t)k * XR, direct, rv);
+  double tot[6];
+  block_totals<NT>(Bk, split ? xblk : nullptr, nblk, b0, b1, direct, tot);
+  if (row0) xrow_store(mol, N, k, rv);
+  if (split) {
+    for (k += NT; k < N; k += NT) {
+      if (k >= i0 && k < i1) continue;
+      xrow_load(xrow + (size_t)k * XR, direct, rv);
+      xrow_store(mol, N, k, rv);
     }
-    PT_MARK(11, pt_sc);
   }
+  PT_MARK(11, pt_sc);
   __syncthreads();
-  double tot[6] = {0, 0, 0, 0, 0, 0};
-  for (int b = threadIdx.x; b < nblk; b += NT) {
-#pragma unroll
-    for (int u = 0; u < 6; u++) tot[u] += Bk[u * nblk + b];
-  }
-  // the next write of S.red follows the barrier that ends second_half (or
-  // the callers' barriers in the other modes)
-  block_sum<NT, 6, false>(tot, S.red);
   c.lj = 0.5 * tot[0];
   c.clmb = 0.5 * tot[1];
   c.wall = tot[2];
