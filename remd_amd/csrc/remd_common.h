@@ -65,6 +65,8 @@ struct JobParams {
                                 // global-state path, any 1..64 on the resident path)
   int precision;                // 0 mixed (FP32 pair, FP64 acc), 1 FP64
   int compact;                  // 1: screen partners by COM distance before the pair terms
+  int radix_min;                // resident path: molecules from which the neighbour sort is the
+                                // wave-ballot radix sort (the rank sort below; same order)
   double dt, dthalf;
   double rcut, rswitch, margin;
   double wall_length;           // A
@@ -96,6 +98,10 @@ struct ReplicaDesc {
 
 // Resident path: threads per workgroup (4 waves: one per SIMD of a CU).
 constexpr int RNT = 256;
+
+// Resident path: default molecule count from which the neighbour sort is
+// the radix sort rather than the rank sort.
+constexpr int RADIX_MIN_DEFAULT = 768;
 
 // Resident path: molecules per energy block.  A replica's parts own whole
 // blocks, and the energy/virial totals are sums of block sums.

@@ -270,6 +270,12 @@ static void fill_params(const remd_gpu_desc *d, JobParams &J) {
   // bulk and slit windows are slabs of the box, mostly beyond the cutoff:
   // screen the partners first; tube windows are mostly inside it
   J.compact = d->confined == 1 ? 0 : 1;
+  // the rank sort's O(N^2 / threads) comparisons lose to the radix sort's
+  // 16 passes from about this size (measured); REMD_SORT_RADIX_MIN overrides
+  {
+    const char *e = std::getenv("REMD_SORT_RADIX_MIN");
+    J.radix_min = e ? std::atoi(e) : RADIX_MIN_DEFAULT;
+  }
   J.dt = d->dt;
   J.dthalf = d->dt * 0.5;
   J.rcut = d->rcut;

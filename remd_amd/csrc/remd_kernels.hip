@@ -575,6 +575,7 @@ __device__ void radix_sort(uint64_t *keyA, uint16_t *idxA, uint64_t *keyB, uint1
   const int ntiles = (n + NT - 1) / NT;
   const int ng = ntiles * NW;
   const uint64_t below = (lane == 0) ? 0ull : ((1ull << lane) - 1ull);
+#pragma unroll 1
   for (int pass = 0; pass < 16; ++pass) {
     const int shift = 4 * pass;
     for (int t = 0; t < ntiles; ++t) {
@@ -1587,6 +1588,28 @@ __device__ bool group_wait(Group &g, int *flag, double *pt_sc = nullptr, bool ce
   return ok;
 }
 
+// Sorted position rk[i] of each key ax[i] (stable: equal keys keep their
+// order) by the wave-ballot radix sort of the keys' order-preserving bit
+// patterns, in `region` (20 N bytes + the digit counts): keys A/B at 0 and
+// 8N, indices A/B at 16N and 18N, counts at 20N; 16 passes leave the result
+// in A.  rk (N 16-bit entries) must lie outside [0, 8N) and [16N, ...).
+// Out of line: inlined into the resident kernel it perturbs the register
+// allocation of the whole step loop.
+template <int NT>
+__device__ __attribute__((noinline)) void radix_ranks(const double *ax, int N, char *region, uint16_t *rk) {
+  uint64_t *keyA = (uint64_t *)region, *keyB = keyA + N;
+  uint16_t *idxA = (uint16_t *)(region + 16 * N), *idxB = idxA + N;
+  int *cnt = (int *)(region + 20 * N);
+  for (int i = threadIdx.x; i < N; i += NT) {
+    const uint64_t u = (uint64_t)__double_as_longlong(ax[i] + 0.0);  // -0.0 -> +0.0
+    keyA[i] = (u >> 63) ? ~u : (u | (1ull << 63));
+    idxA[i] = (uint16_t)i;
+  }
+  __syncthreads();
+  radix_sort<NT>(keyA, idxA, keyB, idxB, cnt, N);
+  for (int k = threadIdx.x; k < N; k += NT) rk[idxA[k]] = (uint16_t)k;
+}
+
 // Stable rank sort of the sort-axis coordinate in LDS (the same order as a
 // stable sort on the double keys: rank = #smaller + #equal-and-earlier),
 // permutation of the state, then the windows by binary search on the sorted
@@ -1598,17 +1621,27 @@ __device__ void rank_sort_resident(JRef J, double *mol, int *mid, int N, double 
   double *z = (double *)S.region0;
   uint16_t *rk = (uint16_t *)(S.region0 + 8 * N);
   double *ax = mol + (F_RX + J.sort_axis) * N;
-  for (int i = threadIdx.x; i < N; i += NT) z[i] = ax[i] + 0.0;  // -0.0 -> +0.0
-  __syncthreads();
-  for (int i = threadIdx.x; i < N; i += NT) {
-    const double zi = z[i];
-    int r = 0;
+  // large replicas: the wave-ballot radix sort of the keys' order-preserving
+  // bit patterns (stable, so the same order as the rank sort), in region 0:
+  // keys A/B at 0 and 8N bytes, indices A/B at 16N and 18N, digit counts
+  // after; 16 passes leave the result in A.  rk then goes to [8N, 10N).
+  const int ng = ((N + NT - 1) / NT) * (NT / WAVE);
+  const bool radix = N >= J.radix_min && 20 * N + 4 * 17 * ng <= resident_region0(N, NS, J.precision);
+  if (radix) {
+    radix_ranks<NT>(ax, N, S.region0, rk);
+  } else {
+    for (int i = threadIdx.x; i < N; i += NT) z[i] = ax[i] + 0.0;  // -0.0 -> +0.0
+    __syncthreads();
+    for (int i = threadIdx.x; i < N; i += NT) {
+      const double zi = z[i];
+      int r = 0;
 #pragma unroll 8
-    for (int j = 0; j < N; ++j) {
-      const double zj = z[j];
-      r += (j < i) ? (zj <= zi ? 1 : 0) : (zj < zi ? 1 : 0);
+      for (int j = 0; j < N; ++j) {
+        const double zj = z[j];
+        r += (j < i) ? (zj <= zi ? 1 : 0) : (zj < zi ? 1 : 0);
+      }
+      rk[i] = (uint16_t)r;
     }
-    rk[i] = (uint16_t)r;
   }
   __syncthreads();
   // permute the state through the larger of the pair-partial area (idle

@@ -426,6 +426,40 @@ def test_large_replica_takes_split_path():
     h.close()
 
 
+@pytest.mark.parametrize("conf", ["tube", "split"])
+def test_resident_radix_sort_equals_rank_sort(conf, monkeypatch):
+    """The resident path's neighbour sort gives the same order with the
+    wave-ballot radix sort as with the rank sort (both stable sorts of the
+    same keys), so two intervals give the same bits in state, molecule ids,
+    windows and averages: a 144-TIP4P tube (state in LDS) and a 700-TIP4P
+    slit (state in per-part global copies)."""
+    from remd_amd.workloads import relax
+
+    if conf == "tube":
+        s = inp.config_tube(seed=71, ntemp=2, npress=2)
+        s = relax(s)
+    else:
+        s = relax(inp.config_slit(seed=61, nmol=700, Lxy=55.0, ntemp=2, npress=2, model="TIP4P"))
+    s.interval = 30
+    out = []
+    for radix_min in (1, 10 ** 6):
+        monkeypatch.setenv("REMD_SORT_RADIX_MIN", str(radix_min))
+        g = ra.REMDGPU(s)
+        g.set_debug(True)
+        for _ in range(2):
+            g.execute_steps(s.T, s.P)
+        out.append((g.launch_path()[0], g.all_averages(), [g.state(k) for k in range(s.nreplica)], g.window(0)))
+        g.close()
+    (pa, aa, sa, wa), (pb, ab, sb, wb) = out
+    assert pa == pb == (ra.PATH_RESIDENT if conf == "tube" else ra.PATH_SPLIT)
+    assert np.all(np.isfinite(aa[:, :15]))
+    assert np.array_equal(aa, ab)
+    assert np.array_equal(wa, wb)
+    for x, y in zip(sa, sb):
+        for k in ("r", "v", "q", "p", "f", "n", "id"):
+            assert np.array_equal(x[k], y[k]), k
+
+
 @pytest.mark.parametrize("path", [ra.PATH_RESIDENT, ra.PATH_SPLIT])
 def test_parts_do_not_change_results(path):
     """A replica split over 7 workgroups (alone) gives the same bits as in a
