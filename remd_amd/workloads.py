@@ -250,5 +250,7 @@ def precision_study(intervals=100, nrep_drift=None):
                                 "replica_steps_per_s": s.nreplica * s.interval * intervals / wall}
     g0.close()
     g1.close()
-    out.update(nmol=s.nmol, replicas=s.nreplica, model=s.model)
+    out.update(nmol=s.nmol, replicas=s.nreplica, model=s.model,
+               wall="slit: two 10-4 sheets (this build's slit wall), not the reference's 9-3 table; "
+                    "errors are against the same kernels in FP64, not comparable with the reference's slit runs")
     return out
