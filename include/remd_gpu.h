@@ -240,6 +240,26 @@ int remd_rng_next(remd_rng *s);
  * number of pairs. */
 int remd_exchange_pairs(int ntemp, int npress, int type, int *a, int *b);
 
+/* nint exchange intervals back to back on the device, with no host round
+ * trip between them: the interval loop of REMD::Run
+ * (src/remd.cxx:241-301: ExecuteSteps, GetAverages, ReplicaExchange,
+ * :486-538) for one job.  Interval k runs at the conditions the exchange
+ * after interval k-1 left (the first at T, P as given, as
+ * remd_gpu_execute_steps does); then the Metropolis exchange of step
+ * step0 + k (pair list (step0 + k) % 4) runs on the device with the
+ * uniforms remd_exchange_host would draw from rng (rand()/RAND_MAX, one per
+ * pair in list order), drawn on the host before the first launch; rng
+ * advances by as many draws.  The same decisions as nint rounds of
+ * remd_gpu_execute_steps + remd_gpu_get_pot_vol + remd_exchange_host.
+ * index (by condition slot), T, P (by replica): in/out.  accepted (may be
+ * NULL): per-pair flags of every interval in order; averages (may be
+ * NULL): nint x nreplica x 19, each interval's GetAverages; kernel_ms (may
+ * be NULL): nint interval kernel times.  Returns the number of accepted
+ * swaps, or a negative error. */
+int remd_gpu_execute_intervals(remd_gpu *g, int ntemp, int npress, long step0, int nint, int *index,
+                               double *T, double *P, remd_rng *rng, int *accepted, double *averages,
+                               double *kernel_ms);
+
 /* REMD::ReplicaExchange (src/remd.cxx:486-538) on the host for one job:
  * pair list step%4 (src/remdinfo.cxx:54-108), Metropolis with one
  * rng draw per pair in list order, REMDInfo::ExchangeAccepted/Rejected

@@ -121,6 +121,9 @@ def lib():
     L.remd_exchange_host.argtypes = [C.c_int, C.c_int, C.c_long, C.c_int, _dp, _dp, _ip, _dp, _dp,
                                      C.POINTER(C.c_ubyte), C.POINTER(Rng), _ip, _ip]
     L.remd_exchange_host.restype = C.c_int
+    L.remd_gpu_execute_intervals.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_long, C.c_int, _ip, _dp, _dp,
+                                             C.POINTER(Rng), _ip, _dp, _dp]
+    L.remd_gpu_execute_intervals.restype = C.c_int
     _lib = L
     return L
 
@@ -333,6 +336,24 @@ class REMDGPU:
         _check(lib().remd_gpu_exchange_device(self.h, int(ntemp), int(npress), int(step), _ip_of(index),
                                               _dp_of(T), _dp_of(P), _dp_of(u), _ip_of(acc), C.byref(npairs)))
         return index, T, P, acc[: npairs.value]
+
+
+def execute_intervals(g: "REMDGPU", ntemp, npress, step0, nint, index, T, P, rng: "GlibcRand"):
+    """remd_gpu_execute_intervals: nint intervals with the exchange on the
+    device between them.  index/T/P are updated in place (int32 / float64);
+    returns (n_accepted, accepted per pair, averages nint x nrep x 19,
+    kernel ms per interval)."""
+    assert index.dtype == np.int32 and T.dtype == np.float64 and P.dtype == np.float64
+    npairs = sum(int(exchange_pairs(ntemp, npress, (step0 + k) % 4)[0].size) for k in range(nint))
+    acc = np.zeros(max(1, npairs), np.int32)
+    av = np.zeros((nint, ntemp * npress, 19))
+    ms = np.zeros(nint)
+    rc = lib().remd_gpu_execute_intervals(g.h, int(ntemp), int(npress), int(step0), int(nint), _ip_of(index),
+                                          _dp_of(T), _dp_of(P), C.byref(rng.s), _ip_of(acc), _dp_of(av),
+                                          _dp_of(ms))
+    if rc < 0:
+        _check(rc)
+    return rc, acc[:npairs], av, ms
 
 
 def exchange_pairs(ntemp, npress, typ):

@@ -142,4 +142,20 @@ struct LaunchArgs {
                                 // acquire (else: stores drained to L2 / device-scope loads)
 };
 
+// One job's device exchange (exchange_kernel): grid, pair list type, the
+// condition arrays and the uniforms in pair-list order.
+struct ExchangeJob {
+  int ntemp, npress, pmode, ec_type;
+  int *index;         // condition slot -> replica
+  double *T, *P;      // per replica conditions (reduced)
+  const double *u;    // uniforms, pair-list order
+  double *const *sc;  // per replica scalar blocks
+  int *accepted;      // per pair, may be null
+  // remd_gpu_execute_intervals: the interval's scalar blocks (nrep x NSC,
+  // contiguous from sc[0]) copied to hist before the exchange, and the
+  // group words of the next interval launch cleared (both may be null)
+  double *hist;
+  unsigned long long *sync;
+};
+
 }  // namespace remd

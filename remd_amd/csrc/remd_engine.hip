@@ -31,14 +31,6 @@ hipError_t launch_wall_slit(int nfwall, double H, double sgm, double eps, double
                             hipStream_t st);
 hipError_t launch_sort_window(const double *keys, int nrow, int n, double w, int *order, int *js, int *je,
                               hipStream_t st);
-struct ExchangeJob {
-  int ntemp, npress, pmode, ec_type;
-  int *index;
-  double *T, *P;
-  const double *u;
-  double *const *sc;
-  int *accepted;
-};
 hipError_t launch_exchange(const ExchangeJob *jobs, int njobs, hipStream_t st);
 hipError_t launch_scratch_setup(const ReplicaDesc *src, ReplicaDesc *dst, int nrep, double *xbase,
                                 size_t xstride, unsigned long long *sync, int nsync, unsigned *blk, int nb,
@@ -145,6 +137,12 @@ struct remd_gpu {
   double **d_scptr = nullptr;
   int *d_acc = nullptr;
   ExchangeJob *d_xjob = nullptr;
+  // remd_gpu_execute_intervals: grow-only buffers and events, kept per job
+  double *xi_u = nullptr, *xi_hist = nullptr;
+  int *xi_acc = nullptr;
+  ExchangeJob *xi_jobs = nullptr;
+  size_t xi_ucap = 0, xi_acap = 0, xi_hcap = 0, xi_jcap = 0;
+  std::vector<hipEvent_t> xi_ev;
 };
 
 // Every resident replica of a batch runs in ONE launch of
@@ -352,6 +350,12 @@ static void free_job(remd_gpu *g) {
   hipFree(g->d_scptr);
   hipFree(g->d_acc);
   hipFree(g->d_xjob);
+  hipFree(g->xi_u);
+  hipFree(g->xi_hist);
+  hipFree(g->xi_acc);
+  hipFree(g->xi_jobs);
+  for (hipEvent_t e : g->xi_ev) hipEventDestroy(e);
+  g->xi_ev.clear();
   hipFree(g->d_sync);
   hipFree(g->d_xbuf);
   hipFree(g->d_gbuf);
@@ -1313,6 +1317,8 @@ extern "C" int remd_gpu_exchange_device(remd_gpu *g, int ntemp, int npress, long
   X.u = g->d_u;
   X.sc = g->d_scptr;
   X.accepted = g->d_acc;
+  X.hist = nullptr;
+  X.sync = nullptr;
   HIPCHK(hipMemcpyAsync(g->d_xjob, &X, sizeof(X), hipMemcpyHostToDevice, g->stream));
   HIPCHK(hipMemcpyAsync(g->d_index, index, sizeof(int) * R, hipMemcpyHostToDevice, g->stream));
   HIPCHK(hipMemcpyAsync(g->d_T, T, sizeof(double) * R, hipMemcpyHostToDevice, g->stream));
@@ -1330,6 +1336,163 @@ extern "C" int remd_gpu_exchange_device(remd_gpu *g, int ntemp, int npress, long
   if (npairs) *npairs = np;
   // keep the host mirror's conditions in step with the device
   return sync_scalars(g);
+}
+
+// GetAverages of one replica's scalar block (Average enum order,
+// src/remdinfo.h:27-41).
+static void averages_of(const double *s, double out[19]) {
+  out[0] = s[S_LJ];
+  out[1] = s[S_CLMB];
+  out[2] = s[S_WALL];
+  out[3] = s[S_VOL];
+  out[4] = s[S_TRA];
+  out[5] = s[S_ROT];
+  out[6] = s[S_TK];
+  out[7] = s[S_PRS3];
+  out[8] = s[S_PRS0];
+  out[9] = s[S_PRS1];
+  out[10] = s[S_PRS2];
+  out[11] = s[S_TOT];
+  out[12] = s[S_HAM];
+  out[13] = s[S_S];
+  out[14] = s[S_PS];
+  out[15] = s[S_PV0];
+  out[16] = s[S_PV1];
+  out[17] = s[S_PV2];
+  out[18] = 0.0;
+}
+
+// Grows a per-job device buffer of remd_gpu_execute_intervals to n elements.
+template <class T>
+static int grow(T **p, size_t *cap, size_t n) {
+  if (n <= *cap) return 0;
+  hipFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  HIPCHK(hipMalloc(p, sizeof(T) * n));
+  *cap = n;
+  return 0;
+}
+
+extern "C" int remd_gpu_execute_intervals(remd_gpu *g, int ntemp, int npress, long step0, int nint, int *index,
+                                          double *T, double *P, remd_rng *rng, int *accepted, double *averages,
+                                          double *kernel_ms) {
+  if (!g || !index || !T || !P || !rng) return fail(REMD_EINVAL, "null argument");
+  if (nint < 1 || step0 < 0) return fail(REMD_EINVAL, "nint must be >= 1 and step0 >= 0");
+  if (ntemp < 1 || npress < 1 || ntemp * npress != g->nrep) return fail(REMD_EINVAL, "ntemp*npress must equal nreplica");
+  const int R = g->nrep;
+  {
+    // index must be a permutation of the replicas: the exchange kernel dereferences it
+    std::vector<char> seen(R, 0);
+    for (int k = 0; k < R; k++) {
+      if (index[k] < 0 || index[k] >= R || seen[index[k]]) return fail(REMD_EINVAL, "index is not a permutation");
+      seen[index[k]] = 1;
+    }
+  }
+  // the uniforms remd_exchange_host would draw, interval after interval
+  std::vector<int> off(nint + 1, 0);
+  for (int k = 0; k < nint; k++) off[k + 1] = off[k] + pair_count(ntemp, npress, (int)((step0 + k) % 4));
+  const int U = off[nint];
+  std::vector<double> u(U > 0 ? U : 1);
+  for (int k = 0; k < U; k++) u[k] = (double)remd_rng_next(rng) / (double)2147483647;  // rand()/RAND_MAX
+  int rc = remd_gpu_set_conditions(g, T, P);
+  if (rc) return rc;
+  if (!g->d_index) {
+    HIPCHK(hipMalloc(&g->d_index, sizeof(int) * R));
+    HIPCHK(hipMalloc(&g->d_T, sizeof(double) * R));
+    HIPCHK(hipMalloc(&g->d_P, sizeof(double) * R));
+    HIPCHK(hipMalloc(&g->d_u, sizeof(double) * (R + 1)));
+    HIPCHK(hipMalloc(&g->d_acc, sizeof(int) * (R + 1)));
+    HIPCHK(hipMalloc(&g->d_scptr, sizeof(double *) * R));
+    HIPCHK(hipMalloc(&g->d_xjob, sizeof(ExchangeJob)));
+    std::vector<double *> ptrs(R);
+    for (int r = 0; r < R; r++) ptrs[r] = g->d_sc + (size_t)r * NSC;
+    HIPCHK(hipMemcpy(g->d_scptr, ptrs.data(), sizeof(double *) * R, hipMemcpyHostToDevice));
+  }
+  if ((rc = grow(&g->xi_u, &g->xi_ucap, (size_t)(U > 0 ? U : 1))) ||
+      (rc = grow(&g->xi_acc, &g->xi_acap, (size_t)(U > 0 ? U : 1))) ||
+      (rc = grow(&g->xi_jobs, &g->xi_jcap, (size_t)nint)) ||
+      (averages && (rc = grow(&g->xi_hist, &g->xi_hcap, (size_t)NSC * R * nint))))
+    return rc;
+  while (g->xi_ev.size() < 2 * (size_t)nint) {
+    hipEvent_t e = nullptr;
+    HIPCHK(hipEventCreate(&e));
+    g->xi_ev.push_back(e);
+  }
+  std::vector<ExchangeJob> X(nint);
+  for (int k = 0; k < nint; k++) {
+    X[k].ntemp = ntemp;
+    X[k].npress = npress;
+    X[k].pmode = g->desc.pconstant;
+    X[k].ec_type = (int)((step0 + k) % 4);
+    X[k].index = g->d_index;
+    X[k].T = g->d_T;
+    X[k].P = g->d_P;
+    X[k].u = g->xi_u + off[k];
+    X[k].sc = g->d_scptr;
+    X[k].accepted = g->xi_acc + off[k];
+    X[k].hist = averages ? g->xi_hist + (size_t)k * NSC * R : nullptr;
+    // the exchange kernel clears the group words for the next launch
+    X[k].sync = (g->resident && k + 1 < nint) ? g->d_sync : nullptr;
+  }
+  HIPCHK(hipMemcpyAsync(g->xi_jobs, X.data(), sizeof(ExchangeJob) * nint, hipMemcpyHostToDevice, g->stream));
+  if (U > 0) HIPCHK(hipMemcpyAsync(g->xi_u, u.data(), sizeof(double) * U, hipMemcpyHostToDevice, g->stream));
+  HIPCHK(hipMemcpyAsync(g->d_index, index, sizeof(int) * R, hipMemcpyHostToDevice, g->stream));
+  HIPCHK(hipMemcpyAsync(g->d_T, T, sizeof(double) * R, hipMemcpyHostToDevice, g->stream));
+  HIPCHK(hipMemcpyAsync(g->d_P, P, sizeof(double) * R, hipMemcpyHostToDevice, g->stream));
+  LaunchArgs a;
+  a.reps = g->d_reps;
+  a.nrep = R;
+  a.mode = MODE_INTERVAL;
+  a.do_sort = 0;
+  a.margin = 0.0;
+  a.parts = g->resident ? pick_parts(g->capacity, R, g->N) : 1;
+  a.same_xcd_ok = same_xcd_barrier_enabled();
+  a.same_xcd_mode = same_xcd_barrier_mode();
+  a.sync = g->d_sync;
+  a.abort = g->d_sync ? (unsigned *)(g->d_sync + R) : nullptr;
+  g->last_parts = a.parts;
+  // the abort word is cleared once: a launch that gives up leaves it set,
+  // every later launch of the call then gives up at once, and the call fails
+  if (a.parts > 1) HIPCHK(hipMemsetAsync(g->d_sync, 0, sizeof(unsigned long long) * (R + 1), g->stream));
+  for (int k = 0; k < nint; k++) {
+    HIPCHK(hipEventRecord(g->xi_ev[2 * k], g->stream));
+    if (g->resident) {
+      rc = launch_res(g, a, g->stream);
+      if (rc) return rc;
+    } else {
+      HIPCHK(launch_interval(a, g->nt, g->NS, g->prm.precision, g->lds, g->stream));
+    }
+    HIPCHK(hipEventRecord(g->xi_ev[2 * k + 1], g->stream));
+    HIPCHK(launch_exchange(g->xi_jobs + k, 1, g->stream));
+  }
+  if (a.parts > 1)
+    HIPCHK(hipMemcpyAsync(g->h_abort, a.abort, sizeof(unsigned), hipMemcpyDeviceToHost, g->stream));
+  std::vector<int> acc(U > 0 ? U : 1);
+  std::vector<double> hist(averages ? (size_t)NSC * R * nint : 0);
+  HIPCHK(hipMemcpyAsync(index, g->d_index, sizeof(int) * R, hipMemcpyDeviceToHost, g->stream));
+  HIPCHK(hipMemcpyAsync(T, g->d_T, sizeof(double) * R, hipMemcpyDeviceToHost, g->stream));
+  HIPCHK(hipMemcpyAsync(P, g->d_P, sizeof(double) * R, hipMemcpyDeviceToHost, g->stream));
+  if (U > 0) HIPCHK(hipMemcpyAsync(acc.data(), g->xi_acc, sizeof(int) * U, hipMemcpyDeviceToHost, g->stream));
+  if (averages)
+    HIPCHK(hipMemcpyAsync(hist.data(), g->xi_hist, sizeof(double) * hist.size(), hipMemcpyDeviceToHost, g->stream));
+  rc = sync_scalars(g);
+  if (rc) return rc;
+  int nacc = 0;
+  for (int k = 0; k < U; k++) {
+    nacc += acc[k];
+    if (accepted) accepted[k] = acc[k];
+  }
+  if (averages)
+    for (int k = 0; k < nint; k++)
+      for (int r = 0; r < R; r++) averages_of(&hist[((size_t)k * R + r) * NSC], averages + ((size_t)k * R + r) * 19);
+  for (int k = 0; k < nint; k++) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, g->xi_ev[2 * k], g->xi_ev[2 * k + 1]));
+    if (kernel_ms) kernel_ms[k] = ms;
+    g->last_ms = ms;
+  }
+  return nacc;
 }
 
 // ----------------------------------------------------------------- batch

@@ -2394,24 +2394,65 @@ __global__ __launch_bounds__(NT) void sort_window_kernel(const double *keys, int
 // src/remdinfo.h:276-359) on device, fed uniforms drawn by the host in
 // pair-list order.  One thread per job; writes the new conditions and the
 // exchange flag into each replica's scalar block.
-struct ExchangeJob {
-  int ntemp, npress, pmode, ec_type;
-  int *index;         // condition slot -> replica
-  double *T, *P;      // per replica conditions (reduced)
-  const double *u;    // uniforms, pair-list order
-  double *const *sc;  // per replica scalar blocks
-  int *accepted;      // per pair, may be null
-};
 
+// Slots (a, b) of pair k of exchange list `type` on a dt x dp grid, in the
+// list order of REMDInfo::ExchangeList (src/remdinfo.cxx:54-108); returns
+// the number of pairs (k beyond it: a = b = -1).
+__device__ inline int exchange_pair(int dt, int dp, int type, int k, int &a, int &b) {
+  int n, i, j;
+  a = b = -1;
+  if (type == 0) {
+    const int w = dt / 2;
+    n = dp * w;
+    if (k >= n || w == 0) return n;
+    j = k / w, i = k - j * w;
+    a = 2 * i + j * dt;
+    b = a + 1;
+  } else if (type == 1) {
+    n = (dp / 2) * dt;
+    if (k >= n) return n;
+    j = k / dt, i = k - j * dt;
+    a = i + 2 * j * dt;
+    b = a + dt;
+  } else if (type == 2) {
+    const int w = dt / 2 - (dt + 1) % 2;
+    n = w > 0 ? dp * w : 0;
+    if (k >= n) return n;
+    j = k / w, i = k - j * w;
+    a = 2 * i + j * dt + 1;
+    b = a + 1;
+  } else {
+    const int h = dp / 2 - (dp + 1) % 2;
+    n = h > 0 ? h * dt : 0;
+    if (k >= n) return n;
+    j = k / dt, i = k - j * dt;
+    a = i + (2 * j + 1) * dt;
+    b = a + dt;
+  }
+  return n;
+}
+
+// REMD::ReplicaExchange (src/remd.cxx:486-538) on the device, one workgroup
+// per job.  The pairs of one list share no slot (so no replica), so they are
+// decided side by side, one thread per pair, each with its own uniform
+// u[k] (pair-list order): the decisions and swaps of the serial loop.
 __global__ void exchange_kernel(const ExchangeJob *jobs, int njobs) {
-  const int jid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int jid = blockIdx.x;
   if (jid >= njobs) return;
   const ExchangeJob &X = jobs[jid];
   const int dt = X.ntemp, dp = X.npress;
   const int nrep = dt * dp;
-  for (int r = 0; r < nrep; r++) X.sc[r][S_XFLAG] = 0.0;
-  int k = 0;
-  auto visit = [&](int a, int b) {
+  if (X.hist)
+    for (int i = threadIdx.x; i < nrep * NSC; i += blockDim.x) X.hist[i] = X.sc[0][i];
+  if (X.sync)
+    for (int r = threadIdx.x; r < nrep; r += blockDim.x) X.sync[r] = 0ull;
+  __syncthreads();
+  for (int r = threadIdx.x; r < nrep; r += blockDim.x) X.sc[r][S_XFLAG] = 0.0;
+  __syncthreads();
+  int a, b;
+  const int np = exchange_pair(dt, dp, X.ec_type, 0, a, b);
+  for (int k = threadIdx.x; k < np; k += blockDim.x) {
+    exchange_pair(dt, dp, X.ec_type, k, a, b);
     const int ind1 = X.index[a], ind2 = X.index[b];
     const double beta1 = 1. / X.T[ind1], beta2 = 1. / X.T[ind2];
     const double pot1 = X.sc[ind1][S_POT], pot2 = X.sc[ind2][S_POT];
@@ -2422,7 +2463,6 @@ __global__ void exchange_kernel(const ExchangeJob *jobs, int njobs) {
     }
     const int acc = exp(-delta) > X.u[k];
     if (X.accepted) X.accepted[k] = acc;
-    k++;
     if (acc) {
       X.sc[ind1][S_XFLAG] = 1.0;
       X.sc[ind2][S_XFLAG] = 1.0;
@@ -2430,25 +2470,10 @@ __global__ void exchange_kernel(const ExchangeJob *jobs, int njobs) {
       t = X.P[ind1]; X.P[ind1] = X.P[ind2]; X.P[ind2] = t;
       X.index[a] = ind2;
       X.index[b] = ind1;
-    } else {
-      X.sc[ind1][S_XFLAG] = 0.0;
-      X.sc[ind2][S_XFLAG] = 0.0;
     }
-  };
-  if (X.ec_type == 0) {
-    for (int j = 0; j < dp; ++j)
-      for (int i = 0; i < dt / 2; ++i) visit(2 * i + j * dt, 2 * i + 1 + j * dt);
-  } else if (X.ec_type == 1) {
-    for (int j = 0; j < dp / 2; ++j)
-      for (int i = 0; i < dt; ++i) visit(i + 2 * j * dt, i + (2 * j + 1) * dt);
-  } else if (X.ec_type == 2) {
-    for (int j = 0; j < dp; ++j)
-      for (int i = 0; i < dt / 2 - (dt + 1) % 2; ++i) visit(2 * i + j * dt + 1, 2 * i + 1 + j * dt + 1);
-  } else {
-    for (int j = 0; j < dp / 2 - (dp + 1) % 2; ++j)
-      for (int i = 0; i < dt; ++i) visit(i + (2 * j + 1) * dt, i + (2 * j + 2) * dt);
   }
-  for (int r = 0; r < nrep; r++) {
+  __syncthreads();
+  for (int r = threadIdx.x; r < nrep; r += blockDim.x) {
     X.sc[r][S_TNEW] = X.T[r];
     X.sc[r][S_PNEW] = X.P[r];
   }
@@ -2632,7 +2657,8 @@ hipError_t launch_scratch_setup(const ReplicaDesc *src, ReplicaDesc *dst, int nr
 }
 
 hipError_t launch_exchange(const ExchangeJob *jobs, int njobs, hipStream_t st) {
-  hipLaunchKernelGGL(exchange_kernel, dim3((njobs + 63) / 64), dim3(64), 0, st, jobs, njobs);
+  if (njobs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(exchange_kernel, dim3(njobs), dim3(256), 0, st, jobs, njobs);
   return hipGetLastError();
 }
 

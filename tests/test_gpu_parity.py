@@ -339,22 +339,85 @@ def test_write_cdv_snapshot(tmp_path, model):
 
 
 # ------------------------------------------------------------------ e1
-def test_exchange_device_matches_oracle():
-    sysm = inp.config_tube(seed=21, nmol=64, Lz=30.0, ntemp=4, npress=2, nfwall=500)
+@pytest.mark.parametrize("nt,npr", [(4, 2), (5, 3), (3, 1), (2, 5), (1, 4)])
+def test_exchange_device_matches_oracle(nt, npr):
+    """The device exchange (one thread per pair of the list) makes the
+    oracle's serial decisions on grids of even and odd sides."""
+    sysm = inp.config_tube(seed=21, nmol=64, Lz=30.0, ntemp=nt, npress=npr, nfwall=500)
     g = ra.REMDGPU(sysm, precision=0)
     pot, vol = g.pot_vol()
     rng = np.random.RandomState(5)
-    index = np.arange(8, dtype=np.int32)
+    n = nt * npr
+    index = np.arange(n, dtype=np.int32)
     T = sysm.T.copy()
     P = sysm.P.copy()
     for step in range(8):
-        u = rng.random_sample(8)
-        i2, T2, P2, acc2, _ = po.replica_exchange(4, 2, step, 2, pot, vol, index, T, P, u)
-        i1, T1, P1, acc1 = g.exchange_device(4, 2, step, index, T, P, u)
+        u = rng.random_sample(n)
+        i2, T2, P2, acc2, _ = po.replica_exchange(nt, npr, step, 2, pot, vol, index, T, P, u)
+        i1, T1, P1, acc1 = g.exchange_device(nt, npr, step, index, T, P, u)
         assert np.array_equal(i1, i2) and np.array_equal(T1, T2) and np.array_equal(P1, P2)
         assert np.array_equal(acc1, acc2[: len(acc1)])
         index, T, P = i1, T1, P1
     g.close()
+
+
+def test_execute_intervals_equals_host_exchange_loop():
+    """remd_gpu_execute_intervals (intervals back to back, the exchange on
+    the device with the uniforms drawn beforehand from the job's rand()
+    stream) gives what the host loop gives: execute_steps, pot/vol,
+    remd_exchange_host, interval after interval -- the same index, T, P and
+    accept flags after every interval, the same per-interval averages and the
+    same final state, bit for bit; and 4 + 4 intervals in two calls equal 8
+    in one."""
+    from remd_amd.workloads import relax
+
+    s = relax(inp.config_tube(seed=77, ntemp=4, npress=2))
+    s.interval = 20
+    # a narrow ladder, so that swaps are accepted
+    T0 = s.T.min() + (s.T - s.T.min()) * 0.05
+    P0 = s.P.copy()
+    K, nt, npr = 8, 4, 2
+
+    g = ra.REMDGPU(s)
+    rng = ra.GlibcRand(1)
+    index = np.arange(s.nreplica, dtype=np.int32)
+    T, P = T0.copy(), P0.copy()
+    host_acc, host_av, host_cond = [], [], []
+    for k in range(K):
+        g.execute_steps(T, P)
+        host_av.append(g.all_averages())
+        pot, vol = g.pot_vol()
+        _, acc, _ = ra.exchange_host(nt, npr, k, s.pconstant, pot, vol, index, T, P, rng)
+        host_acc.append(acc.copy())
+        host_cond.append((index.copy(), T.copy(), P.copy()))
+    host_state = [g.state(r) for r in range(s.nreplica)]
+    host_rng = rng.getstate()
+    g.close()
+    assert sum(int(a.sum()) for a in host_acc) > 0
+
+    for split in ((K,), (4, 4)):
+        g = ra.REMDGPU(s)
+        rng = ra.GlibcRand(1)
+        index = np.arange(s.nreplica, dtype=np.int32)
+        T, P = T0.copy(), P0.copy()
+        acc_all, av_all, step0 = [], [], 0
+        for n in split:
+            nacc, acc, av, ms = ra.execute_intervals(g, nt, npr, step0, n, index, T, P, rng)
+            assert nacc == int(acc.sum()) and ms.shape == (n,) and np.all(ms > 0)
+            acc_all.append(acc)
+            av_all.extend(list(av))
+            step0 += n
+        assert np.array_equal(np.concatenate(acc_all), np.concatenate(host_acc)), split
+        hi, hT, hP = host_cond[-1]
+        assert np.array_equal(index, hi) and np.array_equal(T, hT) and np.array_equal(P, hP), split
+        for k in range(K):
+            assert np.array_equal(av_all[k], host_av[k]), (split, k)
+        for r in range(s.nreplica):
+            st = g.state(r)
+            for name in ("r", "v", "q", "p", "id"):
+                assert np.array_equal(st[name], host_state[r][name]), (split, r, name)
+        assert rng.getstate() == host_rng, split  # as many rand() draws as the host loop
+        g.close()
 
 
 # ------------------------------------------------------------- paths
