@@ -597,6 +597,10 @@ extern "C" int remd_gpu_create(remd_gpu **out, const remd_gpu_desc *d, const rem
       free_job(g);
       return fail(REMD_EINVAL, "chunks must be a power of two on the global-state path");
     }
+    // the default 512 threads need more reduction scratch than 256: a
+    // replica at the LDS limit takes the smaller workgroup
+    if (!d->block_threads && g->nt == 512 && lds_bytes_host(g->N, g->NS, g->prm.precision, 512) > lim)
+      g->nt = 256;
     g->prm.chunks = d->chunks ? d->chunks : pick_chunks(g->N, g->nt, wf);
     g->lds = lds_bytes_host(g->N, g->NS, g->prm.precision, g->nt);
     if (g->lds > lim) {
