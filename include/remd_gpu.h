@@ -42,11 +42,9 @@ enum { REMD_PREC_MIXED = 0, REMD_PREC_FP64 = 1 };
 /* One REMD job: the fields REMDGPU::REMDGPU reads from Molecules/Parameter
  * (src/remdgpu.cu:178-212; src/parameter.h:24-88; src/variables.h:159-207). */
 typedef struct {
-  int nmol;              /* molecules per replica, 2..4096; the replica must also fit
-                            one workgroup's LDS on some path (create fails with
-                            REMD_EINVAL otherwise): with FP32 pair math up to 3051
-                            TIP3P, 2333 TIP4P, 2087 TIP5P (global-state path), with
-                            FP64 pair math about half */
+  int nmol;              /* molecules per replica, 2..4096 (the global-state path keeps
+                            the site records of replicas beyond one workgroup's LDS
+                            in global memory) */
   int nsite;             /* 3, 4 or 5 */
   int nreplica;
   int tconstant;         /* 0 NVE, 1 Nose-Poincare, 2 velocity scaling */

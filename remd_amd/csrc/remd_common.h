@@ -94,6 +94,8 @@ struct ReplicaDesc {
   int *win;                     // optional: 3*nmol window (js, je, whole-ring flag)
   double *xbuf;                 // resident path with parts > 1: exchange rows (xbuf_parity_doubles)
   double *gbuf;                 // split-global path: one private state copy per part (gm_part_doubles)
+  void *grec;                   // global-state path, replicas whose site records do not fit the
+                                // LDS: the records in global memory (nmol x stride x FP32/FP64)
 };
 
 // Resident path: threads per workgroup (4 waves: one per SIMD of a CU).

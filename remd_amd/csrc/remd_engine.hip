@@ -36,7 +36,7 @@ hipError_t launch_scratch_setup(const ReplicaDesc *src, ReplicaDesc *dst, int nr
                                 size_t xstride, unsigned long long *sync, int nsync, unsigned *blk, int nb,
                                 int parts, unsigned variant, hipStream_t st);
 int resident_grid(int nrep, int parts);
-int lds_bytes_host(int nmol, int nsite, int prec, int nt);
+int lds_bytes_host(int nmol, int nsite, int prec, int nt, bool grec = false);
 }  // namespace remd
 
 using namespace remd;
@@ -68,9 +68,9 @@ static double unit_press() { return unit_energy() / (1e-10 * 1e-10 * 1e-10); }
 
 namespace remd {
 static inline int align16h(int x) { return (x + 15) & ~15; }
-int lds_bytes_host(int nmol, int nsite, int prec, int nt) {
+int lds_bytes_host(int nmol, int nsite, int prec, int nt, bool grec) {
   const int stride = (nsite == 5) ? 18 : (nsite == 4 ? 16 : 12);
-  const int rec = nmol * stride * (prec ? 8 : 4);
+  const int rec = grec ? 0 : nmol * stride * (prec ? 8 : 4);
   const int sortb = 20 * nmol;
   const int region0 = align16h(rec > sortb ? rec : sortb);
   const int win = align16h(4 * nmol);
@@ -78,6 +78,13 @@ int lds_bytes_host(int nmol, int nsite, int prec, int nt) {
   return region0 + win + align16h(misc);
 }
 }  // namespace remd
+
+// Bytes of one replica's site records (FP32 or FP64 site positions and
+// body-frame offsets, src/remdgpu.cu:2607-2651 layout stride).
+static size_t rec_bytes(int nmol, int nsite, int prec) {
+  const int stride = (nsite == 5) ? 18 : (nsite == 4 ? 16 : 12);
+  return (size_t)nmol * stride * (prec ? 8 : 4);
+}
 
 static int device_lds_limit() {
   static int lim = -1;
@@ -107,6 +114,8 @@ struct remd_gpu {
   bool resident = false;     // state in LDS for the launch (resident_kernel)
   bool gm = false;           // resident kernel, split-global variant: state in per-part global copies
   double *d_gbuf = nullptr;  // split-global: nrep x GMAX x gm_part_doubles(N)
+  char *d_grec = nullptr;    // global-state path: site records in global memory (rec_bytes each)
+  bool grec = false;         // the global-state path's site records do not fit the LDS
   unsigned *d_blk1 = nullptr;  // split-global: workgroup table of the last (nrep, parts) launch shape
   int blk1_nrep = 0, blk1_parts = 0, blk1_n = 0;
   int capacity = 0;          // resident workgroups the device holds at once
@@ -330,6 +339,8 @@ extern "C" size_t remd_gpu_footprint(const remd_gpu_desc *d) {
   b += R * 2 * xbuf_parity_doubles((int)N) * sizeof(double) + (R + 1) * sizeof(unsigned long long);
   // split-global private copies (replicas of 512 or more molecules may take that path)
   if (N >= 512) b += R * GMAX * gm_part_doubles((int)N) * sizeof(double);
+  // global-state path site records in global memory (replicas beyond the LDS)
+  if (N >= 1024) b += R * rec_bytes((int)N, d->nsite, d->precision);
   return b;
 }
 
@@ -359,6 +370,7 @@ static void free_job(remd_gpu *g) {
   hipFree(g->d_sync);
   hipFree(g->d_xbuf);
   hipFree(g->d_gbuf);
+  hipFree(g->d_grec);
   hipFree(g->d_blk1);
   if (g->h_abort) hipHostFree(g->h_abort);
   if (g->h_sc) hipHostFree(g->h_sc);
@@ -395,6 +407,7 @@ static void refresh_reps(remd_gpu *g) {
     R.win = g->debug ? g->d_win + (size_t)r * g->N * 3 : nullptr;
     R.xbuf = g->d_xbuf ? g->d_xbuf + (size_t)r * 2 * xbuf_parity_doubles(g->N) : nullptr;
     R.gbuf = g->d_gbuf ? g->d_gbuf + (size_t)r * GMAX * gm_part_doubles(g->N) : nullptr;
+    R.grec = g->d_grec ? (void *)(g->d_grec + (size_t)r * rec_bytes(g->N, g->NS, g->prm.precision)) : nullptr;
   }
 }
 
@@ -598,11 +611,16 @@ extern "C" int remd_gpu_create(remd_gpu **out, const remd_gpu_desc *d, const rem
       return fail(REMD_EINVAL, "chunks must be a power of two on the global-state path");
     }
     // the default 512 threads need more reduction scratch than 256: a
-    // replica at the LDS limit takes the smaller workgroup
+    // replica at the LDS limit takes the smaller workgroup; beyond it the
+    // site records move to global memory (read through the caches)
     if (!d->block_threads && g->nt == 512 && lds_bytes_host(g->N, g->NS, g->prm.precision, 512) > lim)
       g->nt = 256;
+    if (lds_bytes_host(g->N, g->NS, g->prm.precision, g->nt) > lim) {
+      g->grec = true;
+      if (!d->block_threads && g->N >= 512) g->nt = 512;
+    }
     g->prm.chunks = d->chunks ? d->chunks : pick_chunks(g->N, g->nt, wf);
-    g->lds = lds_bytes_host(g->N, g->NS, g->prm.precision, g->nt);
+    g->lds = lds_bytes_host(g->N, g->NS, g->prm.precision, g->nt, g->grec);
     if (g->lds > lim) {
       free_job(g);
       return fail(REMD_EINVAL, "replica too large for workgroup LDS (" + std::to_string(g->lds) + " > " +
@@ -638,6 +656,7 @@ extern "C" int remd_gpu_create(remd_gpu **out, const remd_gpu_desc *d, const rem
       HIPCHK(hipHostMalloc(&g->h_abort, sizeof(unsigned), hipHostMallocDefault));
       *g->h_abort = 0u;
     }
+    if (g->grec) HIPCHK(hipMalloc(&g->d_grec, R * rec_bytes((int)N, g->NS, g->prm.precision)));
     return 0;
   }());
   CK(build_wall(g));
@@ -1002,7 +1021,7 @@ extern "C" int remd_gpu_autotune(remd_gpu *g, int reps) {
   const int lim = device_lds_limit();
   for (int a = 0; a < 3; a++) {
     for (int C = 1; C <= 16; C *= 2) {
-      const int lds = lds_bytes_host(g->N, g->NS, g->prm.precision, nts[a]);
+      const int lds = lds_bytes_host(g->N, g->NS, g->prm.precision, nts[a], g->grec);
       if (lds > lim) continue;
       g->nt = nts[a];
       g->lds = lds;
@@ -1026,7 +1045,7 @@ extern "C" int remd_gpu_autotune(remd_gpu *g, int reps) {
   }
   g->nt = bnt;
   g->prm.chunks = bc;
-  g->lds = lds_bytes_host(g->N, g->NS, g->prm.precision, bnt);
+  g->lds = lds_bytes_host(g->N, g->NS, g->prm.precision, bnt, g->grec);
   HIPCHK(hipMemcpyAsync(g->d_job, &g->prm, sizeof(JobParams), hipMemcpyHostToDevice, g->stream));
   // restore the force/energies of the resident state for the chosen shape
   int rc = launch(g, MODE_FORCE, 1, g->prm.margin, g->d_reps, g->nrep, false);

@@ -43,6 +43,7 @@ struct DescD {
   REMD_AS_GLOBAL int *win;
   REMD_AS_GLOBAL double *xbuf;
   REMD_AS_GLOBAL double *gbuf;
+  REMD_AS_GLOBAL char *grec;
 };
 static_assert(sizeof(DescD) == sizeof(ReplicaDesc), "device view of ReplicaDesc");
 template <class T>
@@ -81,9 +82,9 @@ struct Rec {
 __host__ __device__ inline int align16(int x) { return (x + 15) & ~15; }
 
 // Bytes of dynamic LDS a replica of nmol molecules needs.
-__host__ __device__ inline int lds_bytes(int nmol, int nsite, int prec, int nt) {
+__host__ __device__ inline int lds_bytes(int nmol, int nsite, int prec, int nt, bool grec = false) {
   const int stride = (nsite == 5) ? 18 : (nsite == 4 ? 16 : 12);
-  const int rec = nmol * stride * (prec ? 8 : 4);
+  const int rec = grec ? 0 : nmol * stride * (prec ? 8 : 4);
   const int sortb = 20 * nmol;
   int region0 = align16(rec > sortb ? rec : sortb);
   const int win = align16(4 * nmol);
@@ -93,19 +94,21 @@ __host__ __device__ inline int lds_bytes(int nmol, int nsite, int prec, int nt) 
 
 struct Lds {
   char *region0;
+  char *rec;     // site records: region 0, or global memory (grec)
   uint16_t *js, *je;
   double *red;   // NW x 32 doubles
   int *cnt;      // radix counts
 };
 
 template <int NT>
-__device__ inline Lds lds_carve(char *smem, int nmol, int nsite, int prec) {
+__device__ inline Lds lds_carve(char *smem, int nmol, int nsite, int prec, char *grec = nullptr) {
   const int stride = (nsite == 5) ? 18 : (nsite == 4 ? 16 : 12);
-  const int rec = nmol * stride * (prec ? 8 : 4);
+  const int rec = grec ? 0 : nmol * stride * (prec ? 8 : 4);
   const int sortb = 20 * nmol;
   const int region0 = align16(rec > sortb ? rec : sortb);
   Lds L;
   L.region0 = smem;
+  L.rec = grec ? grec : smem;
   L.js = (uint16_t *)(smem + region0);
   L.je = L.js + nmol;
   char *misc = smem + region0 + align16(4 * nmol);
@@ -1020,7 +1023,7 @@ __device__ void force_pass(JRef J, double *mol, int N, Sc &c, const Lds &S,
   PT_DECL
   constexpr int QB = (NS == 3) ? 0 : 1;
   constexpr int ST = Rec<NS>::STRIDE;
-  PT *rec = (PT *)S.region0;
+  PT *rec = (PT *)S.rec;
   m2a<NT, NS, PT>(J, mol, N, c.L[0], c.L[1], c.L[2], rec);
   __syncthreads();
   PT_MARK(4, pt_sc);
@@ -1336,7 +1339,7 @@ __device__ void run_replica(const LaunchArgs &a, const DescD &R, JRef J, char *s
   const int N = J.nmol;
   double *mol = gen(R.mol);
   int *mid = gen(R.mid);
-  Lds S = lds_carve<NT>(smem, N, NS, J.precision);
+  Lds S = lds_carve<NT>(smem, N, NS, J.precision, gen(R.grec));
   Sc c;
   load_sc(c, gen(R.sc));
 

@@ -1,6 +1,6 @@
-"""Edge sizes on the GPU: the largest TIP5P replica that fits a workgroup's
-LDS (2087, bulk) and the smallest replica (2 molecules), one replica each,
-against the oracle;
+"""Edge sizes on the GPU: the ABI's largest replica (4096 TIP5P, bulk), the
+TIP5P replicas either side of the LDS limit of the site records, and the
+smallest replica (2 molecules), one replica each, against the oracle;
 and intervals of a one-replica job (no exchange pairs) through both
 interval entry points.
 
@@ -42,27 +42,51 @@ def _check_step0(s, g):
     assert ef <= 1e-5 and en <= 1e-5
 
 
-# The largest replica is set by the LDS of one workgroup (160 KiB): on the
-# global-state path the FP32 site records (72 B per TIP5P molecule), the
-# window bounds (4 B) and 5184 B of reduction scratch must fit, so 2087 TIP5P.
-TIP5P_MAX = 2087
+# The global-state path keeps the site records in LDS while they fit (FP32
+# records of 72 B per TIP5P molecule + 4 B of window bounds + 5184 B of
+# reduction scratch: up to 2087 TIP5P) and in global memory beyond that, up
+# to the ABI's 4096 molecules.
+TIP5P_LDS_MAX = 2087
+
+
+@pytest.mark.parametrize("n,L", [(TIP5P_LDS_MAX, 39.7), (TIP5P_LDS_MAX + 1, 39.7)])
+def test_tip5p_either_side_of_the_lds_limit(n, L):
+    """2087 TIP5P (records in LDS) and 2088 (records in global memory) on
+    a 13^3 lattice: step-0 forces against the oracle."""
+    s = _bulk(n, L, "TIP5P")
+    g = ra.REMDGPU(s)
+    assert g.launch_path()[0] == ra.PATH_GLOBAL
+    _check_step0(s, g)
+    g.close()
 
 
 @pytest.mark.parametrize("path", [ra.PATH_AUTO, ra.PATH_GLOBAL])
-def test_largest_tip5p_replica(path):
-    """2087 TIP5P on a 13^3 lattice (3.05 A spacing, 39.7 A box), the
-    largest TIP5P replica that fits: step-0 forces against the oracle and a
-    finite interval; one molecule more is refused with a message."""
-    s = _bulk(TIP5P_MAX, 39.7, "TIP5P")
+def test_largest_replica_4096_tip5p(path):
+    """4096 TIP5P on a 16^3 lattice (3.1 A spacing, 49.7 A box), the ABI's
+    largest replica (site records in global memory): step-0 forces against
+    the oracle, then 5 NVE steps of 0.5 fs against the oracle's trajectory
+    (positions within 1e-6 of the box, mixed precision); 4097 molecules are
+    refused.  (The lattice start is far from equilibrium: with the
+    Nose-Poincare thermostat its s runs away within 10 steps, in the oracle
+    too.)"""
+    s = _bulk(4096, 49.7, "TIP5P")
+    s.tconstant, s.dt_fs, s.interval = 0, 0.5, 5
     g = ra.REMDGPU(s, path=path)
     assert g.launch_path()[0] == ra.PATH_GLOBAL
     _check_step0(s, g)
-    s.interval = 10
     g.execute_steps(s.T, s.P)
-    assert np.all(np.isfinite(g.all_averages()[:, :15]))
+    o = oracle_for(s, 0)
+    o.initialize_property()
+    o.execute_steps(5)
+    so, sg = o.state(), g.state(0)
+    a_r, b_r = sg["r"][np.argsort(sg["id"])], so["r"][np.argsort(so["id"])]
+    d = a_r - b_r
+    d -= np.rint(d)
+    print("4096 NVE 5 steps max |dr|", np.max(np.abs(d)))
+    assert np.max(np.abs(d)) <= 1e-6
     g.close()
-    with pytest.raises(RuntimeError, match="too large for workgroup LDS"):
-        ra.REMDGPU(_bulk(TIP5P_MAX + 1, 39.7, "TIP5P"), path=path)
+    with pytest.raises(RuntimeError, match="nmol must be in"):
+        ra.REMDGPU(_bulk(4097, 49.7, "TIP5P"), path=path)
 
 
 @pytest.mark.parametrize("path", [ra.PATH_RESIDENT, ra.PATH_GLOBAL])
