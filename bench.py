@@ -229,7 +229,8 @@ def main():
     ra.execute_intervals(job, system.ntemp, system.npress, step_no, args.steps, index, T, P, rng)
     step_no += args.steps
     path, parts = job.launch_path()
-    kname = ("resident_kernel<256,4,float>" if path == ra.PATH_RESIDENT else
+    # the resident instantiation with the tube NP ensemble fixed (code 19)
+    kname = ("resident_kernel<256,4,float,19>" if path == ra.PATH_RESIDENT else
              f"interval_kernel<{shape['block_threads']},4,float>")
 
     # algorithmic work: ordered in-cutoff pairs at the interval-start snapshot

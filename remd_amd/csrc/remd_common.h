@@ -105,6 +105,20 @@ constexpr int RNT = 256;
 // the radix sort rather than the rank sort.
 constexpr int RADIX_MIN_DEFAULT = 768;
 
+// Ensemble code of a job: tconstant 0..2, pconstant 0..3, confined 0..2.
+constexpr int ens_code(int tconstant, int pconstant, int confined) {
+  return (tconstant * 4 + pconstant) * 3 + confined;
+}
+constexpr int ens_tconstant(int code) { return code / 12; }
+constexpr int ens_pconstant(int code) { return (code / 3) % 4; }
+constexpr int ens_confined(int code) { return code % 3; }
+// Ensembles the single-job resident kernel is also built for with the modes
+// fixed (FP32 pair math): Nose-Poincare thermostat with the 1D barostat in
+// a tube, the area barostat in a slit, the 3D barostat in bulk.
+constexpr int ENS_TUBE_NP = ens_code(1, 2, 1);
+constexpr int ENS_SLIT_NP = ens_code(1, 3, 2);
+constexpr int ENS_BULK_NP = ens_code(1, 1, 0);
+
 // Resident path: molecules per energy block.  A replica's parts own whole
 // blocks, and the energy/virial totals are sums of block sums.
 constexpr int EB = 4;

@@ -21,8 +21,8 @@
 namespace remd {
 hipError_t launch_interval(const LaunchArgs &a, int nt, int nsite, int prec, int lds, hipStream_t st);
 hipError_t set_kernel_lds_limit(int bytes);
-hipError_t launch_resident(const LaunchArgs &a, int nsite, int prec, int lds, hipStream_t st);
-hipError_t resident_occupancy(int nsite, int prec, int lds, int *per_cu);
+hipError_t launch_resident(const LaunchArgs &a, int nsite, int prec, int ens, int lds, hipStream_t st);
+hipError_t resident_occupancy(int nsite, int prec, int ens, int lds, int *per_cu);
 hipError_t launch_resident_batch(const LaunchArgs &a, int lds, hipStream_t st);
 hipError_t resident_batch_occupancy(int lds, int *per_cu);
 int resident_lds(int nmol, int nsite, int prec, int site_dbg, int gm);
@@ -450,13 +450,16 @@ static int pick_parts(int capacity, int nrep, int N) {
   return p >= 2 ? p : 1;
 }
 
+// Ensemble code of the job (selects the resident-kernel instantiation).
+static int job_ens(const remd_gpu *g) { return ens_code(g->prm.tmode, g->prm.pmode, g->prm.cmode); }
+
 // A resident-path launch of nrep replicas with a.parts workgroups each: the
 // LDS kernel, or for the split-global variant the table-driven kernel with
 // the same workgroup mapping (parts of a replica on block indices equal
 // mod 8).
 static int launch_res(remd_gpu *g, LaunchArgs &a, hipStream_t st) {
   if (!g->gm) {
-    HIPCHK(launch_resident(a, g->NS, g->prm.precision, g->lds, st));
+    HIPCHK(launch_resident(a, g->NS, g->prm.precision, job_ens(g), g->lds, st));
     return 0;
   }
   const int parts = a.parts < 1 ? 1 : a.parts;
@@ -588,7 +591,7 @@ extern "C" int remd_gpu_create(remd_gpu **out, const remd_gpu_desc *d, const rem
     int per_cu = 0, ncu = 0, dev = 0;
     hipGetDevice(&dev);
     const hipError_t oe = g->gm ? resident_batch_occupancy(g->lds, &per_cu)
-                                : resident_occupancy(g->NS, g->prm.precision, g->lds, &per_cu);
+                                : resident_occupancy(g->NS, g->prm.precision, job_ens(g), g->lds, &per_cu);
     if (oe != hipSuccess ||
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || per_cu < 1 ||
         ncu < 1) {
@@ -822,7 +825,7 @@ extern "C" int remd_gpu_execute_steps_noalloc(remd_gpu *g, const double *T, cons
     a.nblk = nb;
     HIPCHK(launch_resident_batch(a, g->lds, g->stream));
   } else if (g->resident) {
-    HIPCHK(launch_resident(a, g->NS, g->prm.precision, g->lds, g->stream));
+    HIPCHK(launch_resident(a, g->NS, g->prm.precision, job_ens(g), g->lds, g->stream));
   } else {
     HIPCHK(launch_interval(a, g->nt, g->NS, g->prm.precision, g->lds, g->stream));
   }

@@ -52,6 +52,34 @@ __device__ __forceinline__ T *gen(REMD_AS_GLOBAL T *p) {
 }
 typedef const REMD_AS_CONST JobParams &JRef;
 
+// Ensemble of a resident-path instantiation.  ENS_RUNTIME reads the
+// thermostat, barostat and confinement modes from the job; a code from
+// ens_code() fixes them at compile time.  That removes the branches on the
+// modes (and the registers that hold them) but not one floating-point
+// operation, so both forms give the same bits.
+constexpr int ENS_RUNTIME = -1;
+template <int EM>
+__device__ __forceinline__ int tmode_of(JRef J) {
+  if constexpr (EM < 0) return J.tmode;
+  else return ens_tconstant(EM);
+}
+template <int EM>
+__device__ __forceinline__ int pmode_of(JRef J) {
+  if constexpr (EM < 0) return J.pmode;
+  else return ens_pconstant(EM);
+}
+template <int EM>
+__device__ __forceinline__ int cmode_of(JRef J) {
+  if constexpr (EM < 0) return J.cmode;
+  else return ens_confined(EM);
+}
+// the sort axis follows the confinement: x for the slit, z otherwise
+template <int EM>
+__device__ __forceinline__ int sort_axis_of(JRef J) {
+  if constexpr (EM < 0) return J.sort_axis;
+  else return ens_confined(EM) == 2 ? 0 : 2;
+}
+
 
 
 // Optional phase timers (constant 100 MHz clock), accumulated by thread 0
@@ -197,9 +225,10 @@ __device__ inline void store_sc(const Sc &c, double *sc) {
 }
 
 // Molecules::GetVolume / GetBottomArea (src/molecules.h:357-376)
+template <int EM = ENS_RUNTIME>
 __device__ inline double volume(JRef J, double L0, double L1, double L2) {
-  if (J.cmode == 1) return J.bottom_area * L2;
-  if (J.cmode == 2) return L0 * L1 * J.wall_length;
+  if (cmode_of<EM>(J) == 1) return J.bottom_area * L2;
+  if (cmode_of<EM>(J) == 2) return L0 * L1 * J.wall_length;
   return L0 * L1 * L2;
 }
 
@@ -298,11 +327,13 @@ __device__ inline double rotate(double *q, double *p, int axis, double coef_xi, 
 }
 
 // ------------------------------------------------------------- integrator
-__device__ inline int teff(JRef J) { return J.tmode == 1 ? 1 : 0; }
+template <int EM = ENS_RUNTIME>
+__device__ inline int teff(JRef J) { return tmode_of<EM>(J) == 1 ? 1 : 0; }
 
 // D6 (src/integrator.cxx:634-644)
+template <int EM = ENS_RUNTIME>
 __device__ inline void D6(JRef J, Sc &c) {
-  if (teff(J)) {
+  if (teff<EM>(J)) {
     const double tmp = 1.0 + (c.Ps * (0.5 * c.iQ) * J.dthalf);
     c.s *= tmp * tmp;
     c.Ps /= tmp;
@@ -311,15 +342,16 @@ __device__ inline void D6(JRef J, Sc &c) {
 
 // scalar part of D5 (src/integrator.cxx:581-600); pconstant 3 is the area
 // barostat (Lx = Ly = sqrt(A), conjugate momentum in Pv[0]).
+template <int EM = ENS_RUNTIME>
 __device__ inline void D5_scalars(JRef J, Sc &c) {
-  if (teff(J)) {
+  if (teff<EM>(J)) {
     c.Ps -= c.pot * J.dthalf;
-    if (J.pmode > 0) c.Ps -= (c.P * volume(J, c.L[0], c.L[1], c.L[2])) * J.dthalf;
+    if (pmode_of<EM>(J) > 0) c.Ps -= (c.P * volume<EM>(J, c.L[0], c.L[1], c.L[2])) * J.dthalf;
   }
-  if (J.pmode == 1)
+  if (pmode_of<EM>(J) == 1)
     c.Pv[0] += ((c.vir[0] + c.vir[1] + c.vir[2]) / (3.0 * c.L[0] * c.L[1] * c.L[2]) - c.P) * c.s * J.dthalf;
-  if (J.pmode == 2) c.Pv[2] += c.s * (c.vir[2] / c.L[2] - c.P * J.bottom_area) * J.dthalf;
-  if (J.pmode == 3) {
+  if (pmode_of<EM>(J) == 2) c.Pv[2] += c.s * (c.vir[2] / c.L[2] - c.P * J.bottom_area) * J.dthalf;
+  if (pmode_of<EM>(J) == 3) {
     const double A = c.L[0] * c.L[1];
     c.Pv[0] += c.s * ((c.vir[0] + c.vir[1]) / (2.0 * A) - c.P * J.wall_length) * J.dthalf;
   }
@@ -327,19 +359,20 @@ __device__ inline void D5_scalars(JRef J, Sc &c) {
 
 // D4 (src/integrator.cxx:604-632; pconstant 1 with the Pv^2/(2W) kinetic
 // term of remdgpu.cu:2090-2095)
+template <int EM = ENS_RUNTIME>
 __device__ inline void D4(JRef J, Sc &c) {
-  if (J.pmode == 1) {
-    if (teff(J)) c.Ps -= 0.5 * c.Pv[0] * c.Pv[0] * c.iW * J.dthalf;
-    const double V = volume(J, c.L[0], c.L[1], c.L[2]) + c.Pv[0] * c.s * c.iW * J.dthalf;
+  if (pmode_of<EM>(J) == 1) {
+    if (teff<EM>(J)) c.Ps -= 0.5 * c.Pv[0] * c.Pv[0] * c.iW * J.dthalf;
+    const double V = volume<EM>(J, c.L[0], c.L[1], c.L[2]) + c.Pv[0] * c.s * c.iW * J.dthalf;
     const double l = cbrt(V);
     c.L[0] = c.L[1] = c.L[2] = l;
   }
-  if (J.pmode == 2) {
-    if (teff(J)) c.Ps -= 0.5 * c.Pv[2] * c.Pv[2] * c.iW * J.dthalf;
+  if (pmode_of<EM>(J) == 2) {
+    if (teff<EM>(J)) c.Ps -= 0.5 * c.Pv[2] * c.Pv[2] * c.iW * J.dthalf;
     c.L[2] += c.s * c.Pv[2] * c.iW * J.dthalf;
   }
-  if (J.pmode == 3) {
-    if (teff(J)) c.Ps -= 0.5 * c.Pv[0] * c.Pv[0] * c.iW * J.dthalf;
+  if (pmode_of<EM>(J) == 3) {
+    if (teff<EM>(J)) c.Ps -= 0.5 * c.Pv[0] * c.Pv[0] * c.iW * J.dthalf;
     const double A = c.L[0] * c.L[1] + c.s * c.Pv[0] * c.iW * J.dthalf;
     const double l = sqrt(A);
     c.L[0] = c.L[1] = l;
@@ -367,7 +400,7 @@ __device__ inline void D5_mol(JRef J, const Sc &c, double *v, const double *f,
 
 // ------------------------------------------------------------ properties
 // Molecules::CalcProperties (src/molecules.cxx:702-734) over the replica.
-template <int NT>
+template <int NT, int EM = ENS_RUNTIME>
 __device__ void calc_properties(JRef J, const double *mol, int N, Sc &c, double *sc,
                                 double *red) {
   double acc[4] = {0, 0, 0, 0};
@@ -398,18 +431,18 @@ __device__ void calc_properties(JRef J, const double *mol, int N, Sc &c, double 
   const double tsm = 0.5 * c.Ps * c.Ps / c.Q;
   const double tsp = c.gkT * log(c.s);
   double bst = 0.0;
-  if (J.pmode > 0) {
-    if (J.cmode == 0)
+  if (pmode_of<EM>(J) > 0) {
+    if (cmode_of<EM>(J) == 0)
       bst = 0.5 * (c.Pv[0] * c.Pv[0] + c.Pv[1] * c.Pv[1] + c.Pv[2] * c.Pv[2]) / c.W + c.P * c.L[0] * c.L[1] * c.L[2];
-    else if (J.cmode == 1)
-      bst = 0.5 * c.Pv[2] * c.Pv[2] / c.W + c.P * volume(J, c.L[0], c.L[1], c.L[2]);
+    else if (cmode_of<EM>(J) == 1)
+      bst = 0.5 * c.Pv[2] * c.Pv[2] / c.W + c.P * volume<EM>(J, c.L[0], c.L[1], c.L[2]);
     else
-      bst = 0.5 * c.Pv[0] * c.Pv[0] / c.W + c.P * volume(J, c.L[0], c.L[1], c.L[2]);
+      bst = 0.5 * c.Pv[0] * c.Pv[0] / c.W + c.P * volume<EM>(J, c.L[0], c.L[1], c.L[2]);
   }
   double tot = c.pot + kin;
-  if (J.tmode == 1) tot += tsm + tsp;
-  if (J.pmode > 0) tot += bst;
-  const double V = volume(J, c.L[0], c.L[1], c.L[2]);
+  if (tmode_of<EM>(J) == 1) tot += tsm + tsp;
+  if (pmode_of<EM>(J) > 0) tot += bst;
+  const double V = volume<EM>(J, c.L[0], c.L[1], c.L[2]);
   if (threadIdx.x == 0 && sc) {
     sc[S_TRA0] = tra[0]; sc[S_TRA1] = tra[1]; sc[S_TRA2] = tra[2];
     sc[S_TRA] = tr; sc[S_ROT] = rot; sc[S_KIN] = kin;
@@ -425,7 +458,7 @@ __device__ void calc_properties(JRef J, const double *mol, int N, Sc &c, double 
 }
 
 // Returns tot (same value on every thread) without writing sc.
-template <int NT>
+template <int NT, int EM = ENS_RUNTIME>
 __device__ double total_energy(JRef J, const double *mol, int N, const Sc &c, double *red) {
   double acc[4] = {0, 0, 0, 0};
   const double *I = (const double *)J.inertia;
@@ -451,20 +484,20 @@ __device__ double total_energy(JRef J, const double *mol, int N, const Sc &c, do
   }
   const double kin = tra + acc[3];
   double tot = c.pot + kin;
-  if (J.tmode == 1) tot += 0.5 * c.Ps * c.Ps / c.Q + c.gkT * log(c.s);
-  if (J.pmode > 0) {
-    if (J.cmode == 0)
+  if (tmode_of<EM>(J) == 1) tot += 0.5 * c.Ps * c.Ps / c.Q + c.gkT * log(c.s);
+  if (pmode_of<EM>(J) > 0) {
+    if (cmode_of<EM>(J) == 0)
       tot += 0.5 * (c.Pv[0] * c.Pv[0] + c.Pv[1] * c.Pv[1] + c.Pv[2] * c.Pv[2]) / c.W + c.P * c.L[0] * c.L[1] * c.L[2];
-    else if (J.cmode == 1)
-      tot += 0.5 * c.Pv[2] * c.Pv[2] / c.W + c.P * volume(J, c.L[0], c.L[1], c.L[2]);
+    else if (cmode_of<EM>(J) == 1)
+      tot += 0.5 * c.Pv[2] * c.Pv[2] / c.W + c.P * volume<EM>(J, c.L[0], c.L[1], c.L[2]);
     else
-      tot += 0.5 * c.Pv[0] * c.Pv[0] / c.W + c.P * volume(J, c.L[0], c.L[1], c.L[2]);
+      tot += 0.5 * c.Pv[0] * c.Pv[0] / c.W + c.P * volume<EM>(J, c.L[0], c.L[1], c.L[2]);
   }
   return tot;
 }
 
 // KillMomentum + KillAngularMomentumInTube (src/molecules.cxx:168-210)
-template <int NT>
+template <int NT, int EM = ENS_RUNTIME>
 __device__ void kill_momentum(JRef J, double *mol, int N, const Sc &c, double *red,
                               bool tube_twice) {
   double acc[4] = {0, 0, 0, 0};
@@ -477,9 +510,9 @@ __device__ void kill_momentum(JRef J, double *mol, int N, const Sc &c, double *r
   block_sum<NT, 4>(acc, red);
   const double sum[3] = {acc[0] / acc[3], acc[1] / acc[3], acc[2] / acc[3]};
   for (int i = threadIdx.x; i < N; i += NT) {
-    if (J.cmode == 1) {
+    if (cmode_of<EM>(J) == 1) {
       mol[F_VZ * N + i] -= sum[2];
-    } else if (J.cmode == 2) {
+    } else if (cmode_of<EM>(J) == 2) {
       mol[F_VX * N + i] -= sum[0];
       mol[F_VY * N + i] -= sum[1];
     } else {
@@ -489,7 +522,7 @@ __device__ void kill_momentum(JRef J, double *mol, int N, const Sc &c, double *r
     }
   }
   __syncthreads();
-  const int reps = (J.cmode == 1) ? (tube_twice ? 2 : 1) : 0;
+  const int reps = (cmode_of<EM>(J) == 1) ? (tube_twice ? 2 : 1) : 0;
   for (int rpt = 0; rpt < reps; rpt++) {
     double a2[2] = {0, 0};
     for (int i = threadIdx.x; i < N; i += NT) {
@@ -1204,7 +1237,7 @@ __device__ void force_pass(JRef J, double *mol, int N, Sc &c, const Lds &S,
 // its D5 on each molecule, with the box and s it saw, right before this
 // step's D5 on that molecule -- the same operations in the same order as
 // second_half() followed by first_half(), one pass over the molecules.
-template <int NT, bool FUSE = false>
+template <int NT, bool FUSE = false, int EM = ENS_RUNTIME>
 __device__ void first_half(JRef J, double *mol, int N, Sc &c, double *red) {
   Sc c_prev;  // D5 of the previous step's second half: box and s before its scalars
   if (FUSE) {
@@ -1212,19 +1245,19 @@ __device__ void first_half(JRef J, double *mol, int N, Sc &c, double *red) {
     c_prev.L[0] = c.L[0];
     c_prev.L[1] = c.L[1];
     c_prev.L[2] = c.L[2];
-    D5_scalars(J, c);
-    D6(J, c);
+    D5_scalars<EM>(J, c);
+    D6<EM>(J, c);
   }
-  D6(J, c);
+  D6<EM>(J, c);
   Sc c_old;  // D5 on the molecules uses the box and s from before D4
   c_old.s = c.s;
   c_old.L[0] = c.L[0];
   c_old.L[1] = c.L[1];
   c_old.L[2] = c.L[2];
-  D5_scalars(J, c);
-  D4(J, c);
-  const bool T1 = teff(J) == 1;
-  const bool needPv = (J.pmode > 0) || T1;
+  D5_scalars<EM>(J, c);
+  D4<EM>(J, c);
+  const bool T1 = teff<EM>(J) == 1;
+  const bool needPv = (pmode_of<EM>(J) > 0) || T1;
   double sums[4] = {0, 0, 0, 0};  // sum_Pv[3], sum_Ps
   // PsPv factors (box and s are fixed through the molecule loop)
   // reciprocals of s and of the box once; the factors below are products
@@ -1237,9 +1270,9 @@ __device__ void first_half(JRef J, double *mol, int N, Sc &c, double *red) {
   }
   const double logs = T1 ? log(c.s) : 0.0;
   double fpv = 0.0;
-  if (J.pmode == 1) fpv = c.s / (volume(J, c.L[0], c.L[1], c.L[2]) * 3.0) * J.dthalf;
-  if (J.pmode == 2) fpv = c.s * Li[2] * J.dthalf;
-  if (J.pmode == 3) fpv = c.s / (2.0 * c.L[0] * c.L[1]) * J.dthalf;
+  if (pmode_of<EM>(J) == 1) fpv = c.s / (volume<EM>(J, c.L[0], c.L[1], c.L[2]) * 3.0) * J.dthalf;
+  if (pmode_of<EM>(J) == 2) fpv = c.s * Li[2] * J.dthalf;
+  if (pmode_of<EM>(J) == 3) fpv = c.s / (2.0 * c.L[0] * c.L[1]) * J.dthalf;
   {
     const double coef_xi = 0.25 * si;
     double coef_r[3];
@@ -1292,16 +1325,16 @@ __device__ void first_half(JRef J, double *mol, int N, Sc &c, double *red) {
   if (needPv) {
     for (int d = 0; d < 3; d++) sums[d] *= ipv[d];
   }
-  if (J.pmode == 1) c.Pv[0] += (sums[0] + sums[1] + sums[2]) * fpv;
-  if (J.pmode == 2) c.Pv[2] += sums[2] * fpv;
-  if (J.pmode == 3) c.Pv[0] += (sums[0] + sums[1]) * fpv;
+  if (pmode_of<EM>(J) == 1) c.Pv[0] += (sums[0] + sums[1] + sums[2]) * fpv;
+  if (pmode_of<EM>(J) == 2) c.Pv[2] += sums[2] * fpv;
+  if (pmode_of<EM>(J) == 3) c.Pv[0] += (sums[0] + sums[1]) * fpv;
   if (T1) c.Ps += (0.25 * (sums[0] + sums[1] + sums[2]) + 2.0 * sums[3] - (1.0 + logs) * c.gkT + c.H0) * J.dt;
-  D4(J, c);
+  D4<EM>(J, c);
 }
 
 // Second half (src/integrator.cxx:686-703): D5 on every molecule with the
 // new forces, D5 scalars, D6.  Ends with a workgroup barrier.
-template <int NT>
+template <int NT, int EM = ENS_RUNTIME>
 __device__ void second_half(JRef J, double *mol, int N, Sc &c) {
   for (int i = threadIdx.x; i < N; i += NT) {
     double v[3] = {mol[F_VX * N + i], mol[F_VY * N + i], mol[F_VZ * N + i]};
@@ -1313,8 +1346,8 @@ __device__ void second_half(JRef J, double *mol, int N, Sc &c) {
     mol[F_VX * N + i] = v[0]; mol[F_VY * N + i] = v[1]; mol[F_VZ * N + i] = v[2];
     mol[F_P0 * N + i] = p[0]; mol[F_P1 * N + i] = p[1]; mol[F_P2 * N + i] = p[2]; mol[F_P3 * N + i] = p[3];
   }
-  D5_scalars(J, c);
-  D6(J, c);
+  D5_scalars<EM>(J, c);
+  D6<EM>(J, c);
   __syncthreads();
 }
 
@@ -1618,12 +1651,12 @@ __device__ __attribute__((noinline)) void radix_ranks(const double *ax, int N, c
 // permutation of the state, then the windows by binary search on the sorted
 // keys (src/molecules.cxx:541-585 semantics; raw js/je and the whole-ring
 // flag go to win_dbg when given).
-template <int NT, int NS>
+template <int NT, int NS, int EM = ENS_RUNTIME>
 __device__ void rank_sort_resident(JRef J, double *mol, int *mid, int N, double Laxis,
                                    double margin, const LdsR &S, int *win_dbg) {
   double *z = (double *)S.region0;
   uint16_t *rk = (uint16_t *)(S.region0 + 8 * N);
-  double *ax = mol + (F_RX + J.sort_axis) * N;
+  double *ax = mol + (F_RX + sort_axis_of<EM>(J)) * N;
   // large replicas: the wave-ballot radix sort of the keys' order-preserving
   // bit patterns (stable, so the same order as the rank sort), in region 0:
   // keys A/B at 0 and 8N bytes, indices A/B at 16N and 18N, digit counts
@@ -1719,7 +1752,7 @@ __device__ void window_from_debug(const int *win, int N, const LdsR &S) {
 // block of EB molecules: the lj, clmb, wall, vir[3] sums into Bk (6 x nblk,
 // value-major) and into the global block row xblk when given.  i0 is a
 // block boundary.
-template <int NT, int NS, typename PT>
+template <int NT, int NS, typename PT, int EM = ENS_RUNTIME>
 __device__ void force_range(JRef J, double *mol, int N, const Sc &c, const LdsR &S, int i0,
                             int i1, double *xrow, double *Bk, int nblk, double *xblk, double *site_dbg,
                             double *pt_sc = nullptr) {
@@ -1769,7 +1802,7 @@ __device__ void force_range(JRef J, double *mol, int N, const Sc &c, const LdsR 
   const int C = J.chunks;
   const int B = (NT / C) / EB * EB;  // molecules per round, whole energy blocks
   const int NV = npart(NS, site_dbg != nullptr);
-  const double dr = (J.cmode == 1) ? J.wall_length / (double)J.nfwall : 0.5 * J.wall_length / (double)J.nfwall;
+  const double dr = (cmode_of<EM>(J) == 1) ? J.wall_length / (double)J.nfwall : 0.5 * J.wall_length / (double)J.nfwall;
   const double dri = 1.0 / dr;
   const int t = threadIdx.x;
   for (int ib = i0; ib < i1; ib += B) {
@@ -1789,13 +1822,13 @@ __device__ void force_range(JRef J, double *mol, int N, const Sc &c, const LdsR 
       // wall on the site-0 position (chunk 0 of each molecule), FP64: the
       // table rows are fetched before the pair loop so their latency hides
       // behind it (src/force_calculator.cxx:1494-1531)
-      const bool wall = cc == 0 && (J.cmode == 1 || J.cmode == 2);
+      const bool wall = cc == 0 && (cmode_of<EM>(J) == 1 || cmode_of<EM>(J) == 2);
       double wx = 0.0, wy = 0.0, wz = 0.0, wu = 0.0, f0 = 0.0, f1 = 0.0, e0 = 0.0, e1 = 0.0;
       int widx = 0;
       if (wall) {
         double b0[3];
         body_to_space(q, (const double *)J.site_r[0], b0);
-        if (J.cmode == 1) {
+        if (cmode_of<EM>(J) == 1) {
           wx = (mol[F_RX * N + i] - 0.5) * c.L[0] + b0[0];
           wy = (mol[F_RY * N + i] - 0.5) * c.L[1] + b0[1];
           wu = sqrt(wx * wx + wy * wy);
@@ -1824,7 +1857,7 @@ __device__ void force_range(JRef J, double *mol, int N, const Sc &c, const LdsR 
       if (wall) {
         const double f = (f1 - f0) * dri * (wu - dr * widx) + f0;
         e_wall = (e1 - e0) * dri * (wu - dr * widx) + e0;
-        if (J.cmode == 1) {
+        if (cmode_of<EM>(J) == 1) {
           acc.site[0][0] += f * wx;
           acc.site[0][1] += f * wy;
         } else {
@@ -2038,7 +2071,7 @@ __device__ inline void block_totals(const double *Bk, const double *xblk, int nb
 // virial sums per block of EB molecules -- then the fixed-order reduction of
 // the block sums.  A block is summed in molecule order by one thread and
 // the blocks by a fixed tree, so the totals do not depend on the parts.
-template <int NT, int NS, typename PT>
+template <int NT, int NS, typename PT, int EM = ENS_RUNTIME>
 __device__ bool forces_resident(JRef J, double *mol, int N, Sc &c, const LdsR &S, Group &g,
                                 int i0, int i1, double *site_dbg, double *pt_sc = nullptr) {
   PT_DECL
@@ -2051,7 +2084,7 @@ __device__ bool forces_resident(JRef J, double *mol, int N, Sc &c, const LdsR &S
   double *xrow = g.parts > 1 ? g.xbuf + (size_t)(g.epoch & 1u) * xbuf_parity_doubles(N) : nullptr;
   double *xblk = xrow ? xrow + (size_t)N * XR : nullptr;
   double *Bk = S.E;  // 6 x nblk block sums
-  force_range<NT, NS, PT>(J, mol, N, c, S, i0, i1, xrow, Bk, nblk, xblk, site_dbg, pt_sc);
+  force_range<NT, NS, PT, EM>(J, mol, N, c, S, i0, i1, xrow, Bk, nblk, xblk, site_dbg, pt_sc);
   PT_MARK(5, pt_sc);
   const bool split = g.parts > 1;
   if (split) {
@@ -2091,7 +2124,7 @@ __device__ bool forces_resident(JRef J, double *mol, int N, Sc &c, const LdsR &S
 
 // GM (split-global variant): the replica's state is a private copy per
 // part in global memory (R.gbuf) instead of LDS; everything else as above.
-template <int NT, int NS, typename PT, bool GM = false>
+template <int NT, int NS, typename PT, bool GM = false, int EM = ENS_RUNTIME>
 __device__ __attribute__((always_inline)) void run_resident(const LaunchArgs &a, const DescD &R, JRef J, char *smem,
                              Group &g) {
   const int N = J.nmol;
@@ -2130,22 +2163,22 @@ __device__ __attribute__((always_inline)) void run_resident(const LaunchArgs &a,
   double *site_dbg = gen(R.site_f);
   int *win_dbg = lead ? gen(R.win) : nullptr;
   double *sc_out = lead ? gen(R.sc) : nullptr;
-  const double Laxis0 = J.sort_axis == 0 ? c.L[0] : c.L[2];
+  const double Laxis0 = sort_axis_of<EM>(J) == 0 ? c.L[0] : c.L[2];
 
   if (a.mode == MODE_INIT) {
     // InitializeProperty (src/molecules.h:260-270)
-    rank_sort_resident<NT, NS>(J, mol, mid, N, Laxis0, 0.0, S, win_dbg);
-    if (!forces_resident<NT, NS, PT>(J, mol, N, c, S, g, i0, i1, site_dbg)) return;
+    rank_sort_resident<NT, NS, EM>(J, mol, mid, N, Laxis0, 0.0, S, win_dbg);
+    if (!forces_resident<NT, NS, PT, EM>(J, mol, N, c, S, g, i0, i1, site_dbg)) return;
     __syncthreads();  // S.red is read by forces_resident's last reduction
     c.gkT = 6.0 * (double)N * c.T;
-    c.H0 = total_energy<NT>(J, mol, N, c, S.red);
-    calc_properties<NT>(J, mol, N, c, sc_out, S.red);
+    c.H0 = total_energy<NT, EM>(J, mol, N, c, S.red);
+    calc_properties<NT, EM>(J, mol, N, c, sc_out, S.red);
   } else if (a.mode == MODE_FORCE) {
-    if (a.do_sort) rank_sort_resident<NT, NS>(J, mol, mid, N, Laxis0, a.margin, S, win_dbg);
+    if (a.do_sort) rank_sort_resident<NT, NS, EM>(J, mol, mid, N, Laxis0, a.margin, S, win_dbg);
     else window_from_debug<NT>(gen(R.win), N, S);
-    if (!forces_resident<NT, NS, PT>(J, mol, N, c, S, g, i0, i1, site_dbg)) return;
+    if (!forces_resident<NT, NS, PT, EM>(J, mol, N, c, S, g, i0, i1, site_dbg)) return;
     __syncthreads();  // S.red is read by forces_resident's last reduction
-    calc_properties<NT>(J, mol, N, c, sc_out, S.red);
+    calc_properties<NT, EM>(J, mol, N, c, sc_out, S.red);
   } else {
     if (xflag != 0.0) {
       // REMD::ReplicaExchange for an exchanged replica (src/remd.cxx:526-536)
@@ -2155,40 +2188,40 @@ __device__ __attribute__((always_inline)) void run_resident(const LaunchArgs &a,
         for (int d = 0; d < 4; d++) mol[(F_P0 + d) * N + i] *= coef / c.s;
       }
       c.Ps *= coef;
-      if (J.pmode == 1) c.Pv[0] *= coef * c.s;
-      if (J.pmode == 2) c.Pv[2] *= coef * c.s;
-      if (J.pmode == 3) c.Pv[0] *= coef * c.s;
+      if (pmode_of<EM>(J) == 1) c.Pv[0] *= coef * c.s;
+      if (pmode_of<EM>(J) == 2) c.Pv[2] *= coef * c.s;
+      if (pmode_of<EM>(J) == 3) c.Pv[0] *= coef * c.s;
       c.T = Tn;
       c.gkT = 6.0 * (double)N * Tn;
       c.s = 1.0;
       __syncthreads();
-      kill_momentum<NT>(J, mol, N, c, S.red, false);
+      kill_momentum<NT, EM>(J, mol, N, c, S.red, false);
       c.P = Pn;
-      c.H0 = total_energy<NT>(J, mol, N, c, S.red);
+      c.H0 = total_energy<NT, EM>(J, mol, N, c, S.red);
     }
     // the driver scales once per interval before ExecuteSteps
     // (src/remd.cxx:259-262), which scales again at every step
-    if (J.tmode == 2) velocity_scaling<NT>(J, mol, N, c, S.red);
+    if (tmode_of<EM>(J) == 2) velocity_scaling<NT>(J, mol, N, c, S.red);
     double *pt = lead ? S.ptimer : nullptr;
     (void)pt;
     // the second half of step st - 1 runs fused into the first half of step
     // st (not with per-step velocity scaling, which comes between them)
-    const bool fuse = J.tmode != 2;
+    const bool fuse = tmode_of<EM>(J) != 2;
     for (int st = 0; st < J.interval; ++st) {
       PT_DECL
-      if (J.tmode == 2) velocity_scaling<NT>(J, mol, N, c, S.red);
-      if (fuse && st > 0) first_half<NT, true>(J, mol, N, c, S.red);
-      else first_half<NT>(J, mol, N, c, S.red);
+      if (tmode_of<EM>(J) == 2) velocity_scaling<NT>(J, mol, N, c, S.red);
+      if (fuse && st > 0) first_half<NT, true, EM>(J, mol, N, c, S.red);
+      else first_half<NT, false, EM>(J, mol, N, c, S.red);
       PT_MARK(0, pt);
       if ((st % J.sort_interval) == 0)
-        rank_sort_resident<NT, NS>(J, mol, mid, N, J.sort_axis == 0 ? c.L[0] : c.L[2], J.margin, S, win_dbg);
+        rank_sort_resident<NT, NS, EM>(J, mol, mid, N, sort_axis_of<EM>(J) == 0 ? c.L[0] : c.L[2], J.margin, S, win_dbg);
       PT_MARK(1, pt);
-      if (!forces_resident<NT, NS, PT>(J, mol, N, c, S, g, i0, i1, site_dbg, pt)) return;
+      if (!forces_resident<NT, NS, PT, EM>(J, mol, N, c, S, g, i0, i1, site_dbg, pt)) return;
       PT_MARK(2, pt);
-      if (!fuse || st == J.interval - 1) second_half<NT>(J, mol, N, c);
+      if (!fuse || st == J.interval - 1) second_half<NT, EM>(J, mol, N, c);
       PT_MARK(3, pt);
     }
-    calc_properties<NT>(J, mol, N, c, sc_out, S.red);
+    calc_properties<NT, EM>(J, mol, N, c, sc_out, S.red);
   }
   // every part has read the replica's global state before the lead rewrites it
   if (g.parts > 1 && !group_wait<NT>(g, S.flag)) return;
@@ -2214,7 +2247,7 @@ __device__ __attribute__((always_inline)) void run_resident(const LaunchArgs &a,
 // ceil(nrep / 8) workgroups, the parts of a replica on block indices that
 // are equal mod 8 (workgroups are dispatched to the 8 XCDs round-robin, so a
 // replica's parts share one L2).
-template <int NT, int NS, typename PT>
+template <int NT, int NS, typename PT, int EM = ENS_RUNTIME>
 __global__ __launch_bounds__(NT) void resident_kernel(LaunchArgs a) {
   extern __shared__ __align__(16) char smem[];
   int rep, part;
@@ -2239,7 +2272,7 @@ __global__ __launch_bounds__(NT) void resident_kernel(LaunchArgs a) {
   g.mode = a.same_xcd_mode;
   g.xbuf = gen(R.xbuf);
   g.epoch = 0;
-  run_resident<NT, NS, PT>(a, R, J, smem, g);
+  run_resident<NT, NS, PT, false, EM>(a, R, J, smem, g);
 }
 
 // One launch for every resident replica of a batch, whatever its water
@@ -2254,11 +2287,29 @@ __global__ __launch_bounds__(NT) void resident_kernel(LaunchArgs a) {
 // replica has all its parts resident before a queued replica starts, and
 // no part ever waits on a workgroup that is not yet running.
 // The batch kernel calls the twelve variants as functions (one register
-// allocation each); the single-job kernel inlines its one variant.
-template <int NS, typename PT, bool GM>
+// allocation each), and for FP32 jobs of the ENS_*_NP ensembles the same
+// variants with the modes fixed; the single-job kernel inlines its one
+// variant.
+template <int NS, typename PT, bool GM, int EM = ENS_RUNTIME>
 __device__ __attribute__((noinline)) void run_resident_call(const LaunchArgs &a, const DescD &R, JRef J, char *smem,
                                                             Group &g) {
-  run_resident<RNT, NS, PT, GM>(a, R, J, smem, g);
+  run_resident<RNT, NS, PT, GM, EM>(a, R, J, smem, g);
+}
+
+// The FP32 variants of a batch entry with the ensemble modes fixed at
+// compile time (EM, one of the ENS_*_NP codes).  False for FP64 entries.
+template <int EM>
+__device__ inline bool run_resident_fixed(unsigned variant, const LaunchArgs &a, const DescD &R, JRef J, char *smem,
+                                          Group &g) {
+  switch (variant) {
+    case 0: run_resident_call<3, float, false, EM>(a, R, J, smem, g); return true;
+    case 6: run_resident_call<3, float, true, EM>(a, R, J, smem, g); return true;
+    case 2: run_resident_call<4, float, false, EM>(a, R, J, smem, g); return true;
+    case 8: run_resident_call<4, float, true, EM>(a, R, J, smem, g); return true;
+    case 4: run_resident_call<5, float, false, EM>(a, R, J, smem, g); return true;
+    case 10: run_resident_call<5, float, true, EM>(a, R, J, smem, g); return true;
+    default: return false;
+  }
 }
 
 __global__ __launch_bounds__(RNT) void resident_batch_kernel(LaunchArgs a) {
@@ -2279,7 +2330,12 @@ __global__ __launch_bounds__(RNT) void resident_batch_kernel(LaunchArgs a) {
   g.mode = a.same_xcd_mode;
   g.xbuf = gen(R.xbuf);
   g.epoch = 0;
-  switch ((e >> 8) & 15u) {
+  const unsigned variant = (e >> 8) & 15u;
+  const int ens = ens_code(J.tmode, J.pmode, J.cmode);
+  if (ens == ENS_TUBE_NP && run_resident_fixed<ENS_TUBE_NP>(variant, a, R, J, smem, g)) return;
+  if (ens == ENS_SLIT_NP && run_resident_fixed<ENS_SLIT_NP>(variant, a, R, J, smem, g)) return;
+  if (ens == ENS_BULK_NP && run_resident_fixed<ENS_BULK_NP>(variant, a, R, J, smem, g)) return;
+  switch (variant) {
     case 0: run_resident_call<3, float, false>(a, R, J, smem, g); break;
     case 6: run_resident_call<3, float, true>(a, R, J, smem, g); break;
     case 1: run_resident_call<3, double, false>(a, R, J, smem, g); break;
@@ -2531,8 +2587,22 @@ static hipError_t launch_nt(const LaunchArgs &a, int nsite, int prec, int lds, h
   return hipGetLastError();
 }
 
-// Resident path: block size 256 only.
-static const void *resident_fn(int nsite, int prec) {
+// Resident path: block size 256 only.  The FP32 jobs of the ensembles
+// ENS_*_NP run an instantiation with the modes fixed, the rest the one that
+// reads them from the job (same bits either way).
+static const void *resident_fn(int nsite, int prec, int ens) {
+  if (prec == 0) {
+#define REMD_RES_ENS(E)                                                  \
+  if (ens == E) {                                                        \
+    if (nsite == 3) return (const void *)resident_kernel<RNT, 3, float, E>; \
+    if (nsite == 4) return (const void *)resident_kernel<RNT, 4, float, E>; \
+    if (nsite == 5) return (const void *)resident_kernel<RNT, 5, float, E>; \
+  }
+    REMD_RES_ENS(ENS_TUBE_NP)
+    REMD_RES_ENS(ENS_SLIT_NP)
+    REMD_RES_ENS(ENS_BULK_NP)
+#undef REMD_RES_ENS
+  }
   switch (nsite * 2 + prec) {
     case 6: return (const void *)resident_kernel<RNT, 3, float>;
     case 8: return (const void *)resident_kernel<RNT, 4, float>;
@@ -2550,19 +2620,14 @@ int resident_lds(int nmol, int nsite, int prec, int site_dbg, int gm) {
   return resident_lds_bytes(nmol, nsite, prec, RNT, site_dbg != 0, gm != 0);
 }
 
-hipError_t launch_resident(const LaunchArgs &a, int nsite, int prec, int lds, hipStream_t st) {
+hipError_t launch_resident(const LaunchArgs &a, int nsite, int prec, int ens, int lds, hipStream_t st) {
   if (a.nrep <= 0) return hipSuccess;
-  const dim3 g(resident_grid(a.nrep, a.parts)), b(RNT);
-  switch (nsite * 2 + prec) {
-    case 6: hipLaunchKernelGGL((resident_kernel<RNT, 3, float>), g, b, lds, st, a); break;
-    case 8: hipLaunchKernelGGL((resident_kernel<RNT, 4, float>), g, b, lds, st, a); break;
-    case 10: hipLaunchKernelGGL((resident_kernel<RNT, 5, float>), g, b, lds, st, a); break;
-    case 7: hipLaunchKernelGGL((resident_kernel<RNT, 3, double>), g, b, lds, st, a); break;
-    case 9: hipLaunchKernelGGL((resident_kernel<RNT, 4, double>), g, b, lds, st, a); break;
-    case 11: hipLaunchKernelGGL((resident_kernel<RNT, 5, double>), g, b, lds, st, a); break;
-    default: return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
+  const void *f = resident_fn(nsite, prec, ens);
+  if (!f) return hipErrorInvalidValue;
+  LaunchArgs arg = a;
+  void *args[] = {&arg};
+  const hipError_t e = hipLaunchKernel(f, dim3(resident_grid(a.nrep, a.parts)), dim3(RNT), args, (size_t)lds, st);
+  return e != hipSuccess ? e : hipGetLastError();
 }
 
 hipError_t launch_resident_batch(const LaunchArgs &a, int lds, hipStream_t st) {
@@ -2577,8 +2642,8 @@ hipError_t resident_batch_occupancy(int lds, int *per_cu) {
 }
 
 // Workgroups of the resident kernel one CU holds at once with lds bytes.
-hipError_t resident_occupancy(int nsite, int prec, int lds, int *per_cu) {
-  const void *f = resident_fn(nsite, prec);
+hipError_t resident_occupancy(int nsite, int prec, int ens, int lds, int *per_cu) {
+  const void *f = resident_fn(nsite, prec, ens);
   if (!f) return hipErrorInvalidValue;
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, f, RNT, (size_t)lds);
 }
@@ -2610,9 +2675,11 @@ static hipError_t set_lds_nt(int bytes) {
 
 hipError_t set_kernel_lds_limit(int bytes) {
   hipError_t e = hipSuccess;
-  for (int ns = 3; ns <= 5 && e == hipSuccess; ns++)
-    for (int pr = 0; pr < 2 && e == hipSuccess; pr++)
-      e = hipFuncSetAttribute(resident_fn(ns, pr), hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  const int ens[4] = {ENS_RUNTIME, ENS_TUBE_NP, ENS_SLIT_NP, ENS_BULK_NP};
+  for (int k = 0; k < 4 && e == hipSuccess; k++)
+    for (int ns = 3; ns <= 5 && e == hipSuccess; ns++)
+      for (int pr = 0; pr < 2 && e == hipSuccess; pr++)
+        e = hipFuncSetAttribute(resident_fn(ns, pr, ens[k]), hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   if (e == hipSuccess)
     e = hipFuncSetAttribute((const void *)resident_batch_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                             bytes);
