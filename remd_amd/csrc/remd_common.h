@@ -67,6 +67,8 @@ struct JobParams {
   int compact;                  // 1: screen partners by COM distance before the pair terms
   int radix_min;                // resident path: molecules from which the neighbour sort is the
                                 // wave-ballot radix sort (the rank sort below; same order)
+  int ens_fixed;                // resident path: 1 runs jobs of the ENS_*_NP ensembles on the
+                                // instantiations with the modes fixed (same bits as 0)
   double dt, dthalf;
   double rcut, rswitch, margin;
   double wall_length;           // A
@@ -105,16 +107,19 @@ constexpr int RNT = 256;
 // the radix sort rather than the rank sort.
 constexpr int RADIX_MIN_DEFAULT = 768;
 
-// Ensemble code of a job: tconstant 0..2, pconstant 0..3, confined 0..2.
+// Ensemble code of a job: tconstant 0..2, pconstant 0..3, confined 0..2;
+// ENS_RUNTIME: the modes are read from the job.
+constexpr int ENS_RUNTIME = -1;
 constexpr int ens_code(int tconstant, int pconstant, int confined) {
   return (tconstant * 4 + pconstant) * 3 + confined;
 }
 constexpr int ens_tconstant(int code) { return code / 12; }
 constexpr int ens_pconstant(int code) { return (code / 3) % 4; }
 constexpr int ens_confined(int code) { return code % 3; }
-// Ensembles the single-job resident kernel is also built for with the modes
-// fixed (FP32 pair math): Nose-Poincare thermostat with the 1D barostat in
-// a tube, the area barostat in a slit, the 3D barostat in bulk.
+// Ensembles the resident kernels (single-job and batch) are also built for
+// with the modes fixed (FP32 pair math): Nose-Poincare thermostat with the
+// 1D barostat in a tube, the area barostat in a slit, the 3D barostat in
+// bulk.
 constexpr int ENS_TUBE_NP = ens_code(1, 2, 1);
 constexpr int ENS_SLIT_NP = ens_code(1, 3, 2);
 constexpr int ENS_BULK_NP = ens_code(1, 1, 0);

@@ -283,6 +283,12 @@ static void fill_params(const remd_gpu_desc *d, JobParams &J) {
     const char *e = std::getenv("REMD_SORT_RADIX_MIN");
     J.radix_min = e ? std::atoi(e) : RADIX_MIN_DEFAULT;
   }
+  // REMD_ENSEMBLE_FIXED=0 runs every job on the instantiation that reads
+  // the modes from the job (the tests compare the two)
+  {
+    const char *e = std::getenv("REMD_ENSEMBLE_FIXED");
+    J.ens_fixed = (e && std::atoi(e) == 0) ? 0 : 1;
+  }
   J.dt = d->dt;
   J.dthalf = d->dt * 0.5;
   J.rcut = d->rcut;
@@ -451,7 +457,9 @@ static int pick_parts(int capacity, int nrep, int N) {
 }
 
 // Ensemble code of the job (selects the resident-kernel instantiation).
-static int job_ens(const remd_gpu *g) { return ens_code(g->prm.tmode, g->prm.pmode, g->prm.cmode); }
+static int job_ens(const remd_gpu *g) {
+  return g->prm.ens_fixed ? ens_code(g->prm.tmode, g->prm.pmode, g->prm.cmode) : ENS_RUNTIME;
+}
 
 // A resident-path launch of nrep replicas with a.parts workgroups each: the
 // LDS kernel, or for the split-global variant the table-driven kernel with

@@ -57,7 +57,6 @@ typedef const REMD_AS_CONST JobParams &JRef;
 // ens_code() fixes them at compile time.  That removes the branches on the
 // modes (and the registers that hold them) but not one floating-point
 // operation, so both forms give the same bits.
-constexpr int ENS_RUNTIME = -1;
 template <int EM>
 __device__ __forceinline__ int tmode_of(JRef J) {
   if constexpr (EM < 0) return J.tmode;
@@ -2331,7 +2330,7 @@ __global__ __launch_bounds__(RNT) void resident_batch_kernel(LaunchArgs a) {
   g.xbuf = gen(R.xbuf);
   g.epoch = 0;
   const unsigned variant = (e >> 8) & 15u;
-  const int ens = ens_code(J.tmode, J.pmode, J.cmode);
+  const int ens = J.ens_fixed ? ens_code(J.tmode, J.pmode, J.cmode) : ENS_RUNTIME;
   if (ens == ENS_TUBE_NP && run_resident_fixed<ENS_TUBE_NP>(variant, a, R, J, smem, g)) return;
   if (ens == ENS_SLIT_NP && run_resident_fixed<ENS_SLIT_NP>(variant, a, R, J, smem, g)) return;
   if (ens == ENS_BULK_NP && run_resident_fixed<ENS_BULK_NP>(variant, a, R, J, smem, g)) return;

@@ -670,6 +670,53 @@ def test_batched_job_with_new_conditions_equals_alone(model, nmol):
         g.close()
 
 
+@pytest.mark.parametrize("conf", ["tube", "slit", "bulk"])
+def test_fixed_ensemble_equals_runtime_modes(conf, monkeypatch):
+    """The resident kernels built with the job's thermostat, barostat and
+    confinement modes fixed give the bits of the instantiation that reads
+    them from the job (REMD_ENSEMBLE_FIXED=0), alone and in a batch: a tube
+    (NP, 1D barostat), a slit (NP, area barostat) and a bulk box (NP, 3D
+    barostat), two intervals with the temperatures swapped before the
+    second (the exchange rescale runs inside the launch)."""
+    from remd_amd.workloads import relax
+
+    if conf == "tube":
+        s = inp.config_tube(seed=81, nmol=96, Lz=34.0, ntemp=2, npress=2, nfwall=500)
+    elif conf == "slit":
+        s = inp.config_slit(seed=82, nmol=120, ntemp=2, npress=2, model="TIP5P", Lxy=30.0)
+    else:
+        s = inp.config_bulk(seed=83, nmol=128, ntemp=4, model="TIP3P")
+        s.pconstant = 1
+    s = relax(s)
+    s.interval = 20
+    conds = (s.T.copy(), s.T[::-1].copy())
+    runs = []
+    for fixed in ("1", "0"):
+        monkeypatch.setenv("REMD_ENSEMBLE_FIXED", fixed)
+        g = ra.REMDGPU(s, precision=0)
+        assert g.launch_path()[0] == ra.PATH_RESIDENT
+        for T in conds:
+            g.execute_steps(T, s.P)
+        runs.append((g.all_averages(), [g.state(r) for r in range(s.nreplica)]))
+        g.close()
+        g = ra.REMDGPU(s, precision=0)
+        bt = ra.Batch()
+        bt.add(g)
+        for T in conds:
+            g.set_conditions(T, s.P)
+            bt.execute()
+        runs.append((g.all_averages(), [g.state(r) for r in range(s.nreplica)]))
+        bt.close()
+        g.close()
+    ref_av, ref_st = runs[0]
+    assert np.all(np.isfinite(ref_av[:, :15]))
+    for k, (av, st) in enumerate(runs[1:], 1):
+        assert np.array_equal(av, ref_av), (conf, k)
+        for r in range(s.nreplica):
+            for name in ("r", "v", "q", "p", "f", "n", "id"):
+                assert np.array_equal(st[r][name], ref_st[r][name]), (conf, k, r, name)
+
+
 def test_uneven_autotuned_batch_equals_alone():
     """Four jobs of different sizes, models and confinements in one batch
     for six intervals -- the autotuner times different packing plans on
