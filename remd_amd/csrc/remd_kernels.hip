@@ -1618,7 +1618,9 @@ __device__ bool group_wait(Group &g, int *flag, double *pt_sc = nullptr, bool ce
   const bool ok = *flag != 0;
   if (!g.same_xcd || (g.mode & 2)) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   g.epoch++;
-  __syncthreads();  // flag is rewritten by the next barrier
+  // no barrier needed before flag is rewritten: thread 0 writes it again
+  // only after the workgroup barrier that opens the next call, which every
+  // thread reaches after its read above
   PT_MARK(10, pt_sc);
   return ok;
 }
@@ -1680,20 +1682,30 @@ __device__ void rank_sort_resident(JRef J, double *mol, int *mid, int N, double 
   }
   __syncthreads();
   // permute the state through the larger of the pair-partial area (idle
-  // during the sort) and the rest of region 0, as many fields per round as fit
+  // during the sort) and the rest of region 0, as many fields per round as
+  // fit.  F and n are not permuted: the force pass that follows every sort
+  // rewrites them for every molecule before anything reads them.  Field k of
+  // the NPF permuted ones: r, v (k < 6), then q, p.
+  constexpr int NPF = NFIELD - 7;
   const int cap_part = S.part_n / N;
   const int cap_r0 = (resident_region0(N, NS, J.precision) - align16(10 * N)) / (8 * N);
   double *tmp = cap_part >= cap_r0 ? S.part : (double *)(S.region0 + align16(10 * N));
   const int fit = cap_part >= cap_r0 ? cap_part : cap_r0;
-  const int per = fit < 1 ? 1 : (fit > NFIELD ? NFIELD : fit);
-  for (int f0 = 0; f0 < NFIELD; f0 += per) {
-    const int nf = (f0 + per <= NFIELD) ? per : NFIELD - f0;
+  const int per = fit < 1 ? 1 : (fit > NPF ? NPF : fit);
+  for (int f0 = 0; f0 < NPF; f0 += per) {
+    const int nf = (f0 + per <= NPF) ? per : NPF - f0;
     for (int i = threadIdx.x; i < N; i += NT) {
       const int ri = rk[i];
-      for (int f = 0; f < nf; ++f) tmp[f * N + ri] = mol[(f0 + f) * N + i];
+      for (int f = 0; f < nf; ++f) {
+        const int fld = (f0 + f < 6) ? f0 + f : f0 + f + (F_Q0 - 6);
+        tmp[f * N + ri] = mol[fld * N + i];
+      }
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < nf * N; k += NT) mol[f0 * N + k] = tmp[k];
+    for (int f = 0; f < nf; ++f) {
+      const int fld = (f0 + f < 6) ? f0 + f : f0 + f + (F_Q0 - 6);
+      for (int k = threadIdx.x; k < N; k += NT) mol[fld * N + k] = tmp[f * N + k];
+    }
     __syncthreads();
   }
   {
