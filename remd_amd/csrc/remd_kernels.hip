@@ -72,6 +72,12 @@ __device__ __forceinline__ int cmode_of(JRef J) {
   if constexpr (EM < 0) return J.cmode;
   else return ens_confined(EM);
 }
+// periodic axes follow the confinement (bit d: axis d; x, y not in the
+// tube, z not in the slit); -1: read from the job
+template <int EM>
+__device__ __host__ constexpr int per_mask_of() {
+  return EM < 0 ? -1 : (ens_confined(EM) == 1 ? 4 : (ens_confined(EM) == 2 ? 3 : 7));
+}
 // the sort axis follows the confinement: x for the slit, z otherwise
 template <int EM>
 __device__ __forceinline__ int sort_axis_of(JRef J) {
@@ -790,7 +796,9 @@ struct PairK {
   PT sig2, eps4, eps48;
   PT qq[NS][NS];
   PT L[3], Li[3];  // L: the box length on periodic axes, 0 on the others (no wrap there)
+  bool per[3];     // periodic axes (the minimum image applies)
 };
+
 
 template <int NS, typename AT = double>
 struct Acc {
@@ -807,6 +815,15 @@ __device__ inline double rcp(double x) { return 1.0 / x; }
 __device__ inline float rnd(float x) { return rintf(x); }
 __device__ inline double rnd(double x) { return rint(x); }
 
+// Minimum image of a COM difference on axis d: on the periodic axes only.
+// PER is the mask of periodic axes when the ensemble fixes them at compile
+// time, -1 when they are read from the job; both give the same bits.
+template <int PER, int NS, typename PT>
+__device__ __forceinline__ PT min_image(PT x, const PairK<NS, PT> &k, int d) {
+  if constexpr (PER < 0) return k.per[d] ? x - k.L[d] * rnd(x * k.Li[d]) : x;
+  else return ((PER >> d) & 1) ? x - k.L[d] * rnd(x * k.Li[d]) : x;
+}
+
 // One ordered molecule pair (i, j): switched LJ (site 0 - site 0) + Coulomb
 // over charged site pairs, CPU tuned-kernel semantics
 // (src/force_calculator.cxx:874-989).  Pair math in PT, accumulation in AT.
@@ -814,15 +831,12 @@ __device__ inline double rnd(double x) { return rint(x); }
 // the CPU kernel's r02 = min(rc^2, r02) does, :883), so every sum receives
 // a signed zero and keeps its bits; `self` (j == i) moves the partner far
 // outside the cutoff first, so that every term stays finite.
-template <int NS, int QB, typename PT, typename AT>
+template <int NS, int QB, typename PT, int PER, typename AT>
 __device__ __forceinline__ void pair_acc(const PT *gi, const PT (&ai)[NS][3], const PT *rj,
                                          const PairK<NS, PT> &k, bool self, Acc<NS, AT> &acc) {
   PT dg[3];
 #pragma unroll
-  for (int d = 0; d < 3; d++) {
-    dg[d] = gi[d] - rj[d];
-    dg[d] -= k.L[d] * rnd(dg[d] * k.Li[d]);
-  }
+  for (int d = 0; d < 3; d++) dg[d] = min_image<PER>(gi[d] - rj[d], k, d);
   dg[2] = self ? PT(1.0e4) : dg[2];
   const PT r2 = dg[0] * dg[0] + dg[1] * dg[1] + dg[2] * dg[2];
   const bool inside = r2 < k.rc2;
@@ -913,7 +927,7 @@ __device__ __forceinline__ void load_rec(PT (&r)[3 + 3 * NS], const PT *src) {
 // (radius slightly above the cutoff) and only the ones inside go through
 // pair_acc, still in order: the same pairs and the same sums, with less
 // work where most partners lie beyond the cutoff (bulk and slit windows).
-template <int NS, int QB, typename PT, typename AT>
+template <int NS, int QB, typename PT, int PER, typename AT>
 __device__ __forceinline__ void pair_window_run(const PT *rec, int N, int i, int js, int je, int c0, int C,
                                                 bool compact, const PT *gi, const PT (&ai)[NS][3],
                                                 const PairK<NS, PT> &k, Acc<NS, AT> &acc) {
@@ -931,7 +945,7 @@ __device__ __forceinline__ void pair_window_run(const PT *rec, int N, int i, int
       const int jx = more ? ((jn >= N) ? jn - N : jn) : j;
       PT rn[RL];
       load_rec<NS, PT>(rn, rec + jx * ST);
-      pair_acc<NS, QB, PT>(gi, ai, rj, k, j == i, acc);
+      pair_acc<NS, QB, PT, PER>(gi, ai, rj, k, j == i, acc);
       if (!more) break;
       jj = jn;
       j = jx;
@@ -953,10 +967,7 @@ __device__ __forceinline__ void pair_window_run(const PT *rec, int N, int i, int
         const PT cn[3] = {rec[jx * ST], rec[jx * ST + 1], rec[jx * ST + 2]};
         PT dg[3];
 #pragma unroll
-        for (int d = 0; d < 3; d++) {
-          dg[d] = gi[d] - c[d];
-          dg[d] -= k.L[d] * rnd(dg[d] * k.Li[d]);
-        }
+        for (int d = 0; d < 3; d++) dg[d] = min_image<PER>(gi[d] - c[d], k, d);
         const PT r2 = dg[0] * dg[0] + dg[1] * dg[1] + dg[2] * dg[2];
         if (j != i && r2 < k.rs2) mask |= 1u << u;
         if (!more) break;
@@ -984,7 +995,7 @@ __device__ __forceinline__ void pair_window_run(const PT *rec, int N, int i, int
       }
       PT rn[RL];
       load_rec<NS, PT>(rn, rec + jx * ST);
-      pair_acc<NS, QB, PT>(gi, ai, rj, k, false, acc);
+      pair_acc<NS, QB, PT, PER>(gi, ai, rj, k, false, acc);
       if (!more) break;
       j = jx;
 #pragma unroll
@@ -998,7 +1009,7 @@ __device__ __forceinline__ void pair_window_run(const PT *rec, int N, int i, int
 // pair math sums in FP64.  The chunks are then reduced in FP64.  A chunk's
 // partners do not depend on how a replica is split over workgroups, so the
 // results stay the same bits for any split.
-template <int NS, int QB, typename PT>
+template <int NS, int QB, typename PT, int PER = -1>
 __device__ __forceinline__ void pair_window(const PT *rec, int N, int i, int js, int je, int c0, int C,
                                             bool compact, const PT *gi, const PT (&ai)[NS][3],
                                             const PairK<NS, PT> &k, Acc<NS> &acc) {
@@ -1009,7 +1020,7 @@ __device__ __forceinline__ void pair_window(const PT *rec, int N, int i, int js,
     part.corr[0] = part.corr[1] = part.corr[2] = 0.0f;
     part.vir[0] = part.vir[1] = part.vir[2] = 0.0f;
     part.lj = part.clmb = 0.0f;
-    pair_window_run<NS, QB, PT>(rec, N, i, js, je, c0, C, compact, gi, ai, k, part);
+    pair_window_run<NS, QB, PT, PER>(rec, N, i, js, je, c0, C, compact, gi, ai, k, part);
 #pragma unroll
     for (int a = 0; a < NS; a++)
 #pragma unroll
@@ -1022,7 +1033,7 @@ __device__ __forceinline__ void pair_window(const PT *rec, int N, int i, int js,
     acc.lj = (double)part.lj;
     acc.clmb = (double)part.clmb;
   } else {
-    pair_window_run<NS, QB, PT>(rec, N, i, js, je, c0, C, compact, gi, ai, k, acc);
+    pair_window_run<NS, QB, PT, PER>(rec, N, i, js, je, c0, C, compact, gi, ai, k, acc);
   }
 }
 
@@ -1096,6 +1107,7 @@ __device__ void force_pass(JRef J, double *mol, int N, Sc &c, const Lds &S,
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     k.L[d] = J.per[d] ? (PT)c.L[d] : PT(0);
+    k.per[d] = J.per[d] != 0;
     k.Li[d] = (PT)(1.0 / c.L[d]);
   }
 
@@ -1667,18 +1679,41 @@ __device__ void rank_sort_resident(JRef J, double *mol, int *mid, int N, double 
   if (radix) {
     radix_ranks<NT>(ax, N, S.region0, rk);
   } else {
-    for (int i = threadIdx.x; i < N; i += NT) z[i] = ax[i] + 0.0;  // -0.0 -> +0.0
-    __syncthreads();
+    // rank = #smaller + #equal-and-earlier, the N x N comparisons split into
+    // (molecule, segment of partners) items so that every thread takes part:
+    // the segment count minimises the longest thread's comparisons; the
+    // segments' counts meet in an integer per molecule (order-free)
+    int *cnt = (int *)(S.region0 + align16(10 * N));
+    int nseg = 1, best = 1 << 30;
+    for (int g = 1; g <= 8; g++) {
+      const int len = (N + g - 1) / g;
+      const int cost = ((N * g + NT - 1) / NT) * len + 8 * g;
+      if (cost < best) {
+        best = cost;
+        nseg = g;
+      }
+    }
+    const int len = (N + nseg - 1) / nseg;
     for (int i = threadIdx.x; i < N; i += NT) {
+      z[i] = ax[i] + 0.0;  // -0.0 -> +0.0
+      cnt[i] = 0;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < N * nseg; t += NT) {
+      const int i = t % N, seg = t / N;
       const double zi = z[i];
+      const int j0 = seg * len, j1 = min(N, j0 + len);
       int r = 0;
 #pragma unroll 8
-      for (int j = 0; j < N; ++j) {
+      for (int j = j0; j < j1; ++j) {
         const double zj = z[j];
         r += (j < i) ? (zj <= zi ? 1 : 0) : (zj < zi ? 1 : 0);
       }
-      rk[i] = (uint16_t)r;
+      if (nseg == 1) cnt[i] = r;
+      else atomicAdd(&cnt[i], r);
     }
+    __syncthreads();
+    for (int i = threadIdx.x; i < N; i += NT) rk[i] = (uint16_t)cnt[i];
   }
   __syncthreads();
   // permute the state through the larger of the pair-partial area (idle
@@ -1808,6 +1843,7 @@ __device__ void force_range(JRef J, double *mol, int N, const Sc &c, const LdsR 
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     k.L[d] = J.per[d] ? (PT)c.L[d] : PT(0);
+    k.per[d] = J.per[d] != 0;
     k.Li[d] = (PT)(1.0 / c.L[d]);
   }
   const int C = J.chunks;
@@ -1863,7 +1899,8 @@ __device__ void force_range(JRef J, double *mol, int N, const Sc &c, const LdsR 
         ai[a][1] = ri[4 + 3 * a];
         ai[a][2] = ri[5 + 3 * a];
       }
-      pair_window<NS, QB, PT>(rec, N, i, S.js[i], S.je[i], cc, C, J.compact != 0, gi, ai, k, acc);
+      pair_window<NS, QB, PT, per_mask_of<EM>()>(rec, N, i, S.js[i], S.je[i], cc, C, J.compact != 0, gi, ai, k,
+                                              acc);
       double e_wall = 0.0;
       if (wall) {
         const double f = (f1 - f0) * dri * (wu - dr * widx) + f0;
