@@ -796,9 +796,7 @@ struct PairK {
   PT sig2, eps4, eps48;
   PT qq[NS][NS];
   PT L[3], Li[3];  // L: the box length on periodic axes, 0 on the others (no wrap there)
-  bool per[3];     // periodic axes (the minimum image applies)
 };
-
 
 template <int NS, typename AT = double>
 struct Acc {
@@ -815,12 +813,14 @@ __device__ inline double rcp(double x) { return 1.0 / x; }
 __device__ inline float rnd(float x) { return rintf(x); }
 __device__ inline double rnd(double x) { return rint(x); }
 
-// Minimum image of a COM difference on axis d: on the periodic axes only.
-// PER is the mask of periodic axes when the ensemble fixes them at compile
-// time, -1 when they are read from the job; both give the same bits.
+// Minimum image of a COM difference on axis d.  PER < 0: every axis, with
+// L = 0 on the non-periodic ones (x - 0 * round(x / L) == x); PER >= 0: the
+// mask of periodic axes fixed by the ensemble at compile time, and the
+// non-periodic axes skip the no-op.  Both give the same value (the signed
+// zero of an exactly zero difference aside, which no later result keeps).
 template <int PER, int NS, typename PT>
 __device__ __forceinline__ PT min_image(PT x, const PairK<NS, PT> &k, int d) {
-  if constexpr (PER < 0) return k.per[d] ? x - k.L[d] * rnd(x * k.Li[d]) : x;
+  if constexpr (PER < 0) return x - k.L[d] * rnd(x * k.Li[d]);
   else return ((PER >> d) & 1) ? x - k.L[d] * rnd(x * k.Li[d]) : x;
 }
 
@@ -1107,7 +1107,6 @@ __device__ void force_pass(JRef J, double *mol, int N, Sc &c, const Lds &S,
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     k.L[d] = J.per[d] ? (PT)c.L[d] : PT(0);
-    k.per[d] = J.per[d] != 0;
     k.Li[d] = (PT)(1.0 / c.L[d]);
   }
 
@@ -1843,7 +1842,6 @@ __device__ void force_range(JRef J, double *mol, int N, const Sc &c, const LdsR 
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     k.L[d] = J.per[d] ? (PT)c.L[d] : PT(0);
-    k.per[d] = J.per[d] != 0;
     k.Li[d] = (PT)(1.0 / c.L[d]);
   }
   const int C = J.chunks;
