@@ -55,8 +55,8 @@ typedef const REMD_AS_CONST JobParams &JRef;
 // Ensemble of a resident-path instantiation.  ENS_RUNTIME reads the
 // thermostat, barostat and confinement modes from the job; a code from
 // ens_code() fixes them at compile time.  That removes the branches on the
-// modes (and the registers that hold them) but not one floating-point
-// operation, so both forms give the same bits.
+// modes (and the registers that hold them) and the minimum image on axes
+// that are not periodic (a no-op there), so both forms give the same bits.
 template <int EM>
 __device__ __forceinline__ int tmode_of(JRef J) {
   if constexpr (EM < 0) return J.tmode;
@@ -814,10 +814,11 @@ __device__ inline float rnd(float x) { return rintf(x); }
 __device__ inline double rnd(double x) { return rint(x); }
 
 // Minimum image of a COM difference on axis d.  PER < 0: every axis, with
-// L = 0 on the non-periodic ones (x - 0 * round(x / L) == x); PER >= 0: the
-// mask of periodic axes fixed by the ensemble at compile time, and the
-// non-periodic axes skip the no-op.  Both give the same value (the signed
-// zero of an exactly zero difference aside, which no later result keeps).
+// L = 0 on the non-periodic ones; PER >= 0: the mask of periodic axes fixed
+// by the ensemble at compile time, and the non-periodic axes skip the
+// no-op.  x - 0 * round(x / L) is x bit for bit for every finite x (a zero
+// difference of two centred coordinates is +0, and +0 - 0 * 0 == +0), so
+// both give the same bits.
 template <int PER, int NS, typename PT>
 __device__ __forceinline__ PT min_image(PT x, const PairK<NS, PT> &k, int d) {
   if constexpr (PER < 0) return x - k.L[d] * rnd(x * k.Li[d]);
